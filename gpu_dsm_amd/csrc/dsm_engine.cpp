@@ -1,0 +1,1033 @@
+// dsm_engine.cpp -- host side of the C ABI (include/dsm.h): ensembles, HBM arenas,
+// staging windows, Z-binned batched launches and the correlator / flow-stress
+// bookkeeping.  Compiled by hipcc for gfx950 together with dsm_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dsm.h"
+#include "dsm_launch.h"
+#include "dsm_types.h"
+
+using namespace dsm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                       \
+  do {                                                                                  \
+    hipError_t _e = (x);                                                                \
+    if (_e != hipSuccess) return fail(DSM_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr int kSlots[5] = {1, 2, 4, 8, 16};
+constexpr int kNumBins = 5;
+constexpr int kDefaultWindow = 256;
+
+// ---------------------------------------------------------------- p_CD (FP32 host, as the reference)
+// common/pcd_tau.h:32-63 and the device constants of common/gpu_random.cu:76-99.
+struct Pcd {
+  float g, alpha, tau_0, tau_max, tau_d, c1, cm1, rcm1c1, At, Adt, Bdt, normdt;
+  float s_At, s_Ct, s_Dt, s_Adt, s_Bdt, s_Cdt, s_Ddt, s_taud_inv, s_one_minus_g;
+};
+
+void pcd_init(Pcd& p, float Be, float Nk) {
+  const double z = (Nk + Be) / (Be + 1.0);
+  p.g = 0.667f;
+  if (Be != 1.0f) {
+    p.alpha = (0.053f * logf(Be) + 0.31f) * powf((float)z, -0.012f * logf(Be) - 0.024f);
+    p.tau_0 = 0.285f * powf(Be + 2.0f, 0.515f);
+    p.tau_max = ((Nk < 2) ? p.tau_0 : 0.025f * powf(Be + 2.0f, 2.6f) * powf((float)z, 2.83f));
+    p.tau_d = ((Nk < 2) ? p.tau_0 : 0.036f * powf(Be + 2.0f, 3.07f) * powf((float)(z - 1.0f), 3.02f));
+  } else {
+    p.alpha = 0.267096f - 0.375571f * expf(-0.0838237f * Nk);
+    p.tau_0 = 0.460277f + 0.298913f * expf(-0.0705314f * Nk);
+    p.tau_max = ((Nk < 4) ? p.tau_0 : 0.0156137f * powf(Nk, 3.18849f));
+    p.tau_d = ((Nk < 4) ? p.tau_0 : 0.0740131f * powf(Nk, 3.18363f));
+  }
+  p.c1 = powf(p.tau_max, p.alpha) - powf(p.tau_0, p.alpha);
+  p.cm1 = powf(p.tau_max, p.alpha - 1.0f) - powf(p.tau_0, p.alpha - 1.0f);
+  p.rcm1c1 = ((p.c1 == 0.0f) ? (p.alpha - 1.0f) / p.alpha / p.tau_0 : (p.cm1 / p.c1));
+  p.At = (1.0f - p.g);
+  p.Adt = p.At * p.alpha / (p.alpha - 1.0f);
+  p.Bdt = p.Adt * p.rcm1c1;
+  p.normdt = p.Bdt + p.g / p.tau_d;
+  p.s_taud_inv = 1.0f / p.tau_d;
+  p.s_At = 1.0f * p.c1 / p.At;
+  p.s_Dt = powf(p.tau_0, p.alpha);
+  p.s_Ct = -1.0f / p.alpha;
+  p.s_Adt = p.normdt * p.c1 / p.Adt;
+  p.s_Bdt = p.Bdt / p.normdt;
+  p.s_Cdt = -1.0f / (p.alpha - 1.0f);
+  p.s_Ddt = powf(p.tau_0, p.alpha - 1.0f);
+  p.s_one_minus_g = 1.0f - p.g;
+}
+
+double pow075(double x) { return std::sqrt(x) * std::sqrt(std::sqrt(x)); }
+
+// FP32 rate tables, indexed by the integer strand length N (see dsm_kernels.hip pair_rate).
+void build_tables(std::vector<float>& t, int nmax, float beta, int cd, float c_create) {
+  const int n = nmax + 2;
+  const int fr = 4 * nmax + 4;
+  t.assign((size_t)kNumTables * n + fr, 0.0f);
+  float* SP = &t[0]; float* SM = SP + n; float* PP = SM + n; float* PM = PP + n; float* EC = PM + n;
+  float* ED = EC + n; float* ED2 = ED + n; float* T3 = ED2 + n; float* CC = T3 + n; float* FR = CC + n;
+  for (int N = 1; N < n; ++N) {
+    const float fN = (float)N;
+    SP[N] = 0.75f / (fN * (float)(N + 1));
+    SM[N] = N >= 2 ? 0.75f / (fN * (float)(N - 1)) : 0.0f;
+    PP[N] = (float)pow075((double)N / (double)(N + 1));
+    PM[N] = N >= 2 ? (float)pow075((double)N / (double)(N - 1)) : 0.0f;
+    EC[N] = 2.0f / (beta * (fN + 0.5f));
+    ED[N] = 1.0f / (0.75f + 0.5f * fN);
+    ED2[N] = 1.0f / (0.75f + (fN + 0.25f));
+    T3[N] = 3.0f / fN;
+    CC[N] = cd ? c_create * (float)(N - 1) : 0.0f;
+  }
+  for (int x = 1; x < fr; ++x) FR[x] = 4.0f / (float)x;
+}
+
+int slots_for(int nmax) {  // smallest S with 64*S > nmax (room for one insertion)
+  for (int b = 0; b < kNumBins; ++b)
+    if (64 * kSlots[b] > nmax) return b;
+  return -1;
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(DSM_ERR_MEMORY, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return DSM_OK;
+}
+
+struct Ens {
+  bool live = false;
+  dsm_ensemble_desc d{};
+  std::vector<int> nk;
+  int chain0 = 0, n = 0, nmax = 0;
+  int64_t strand0 = 0, nstrand = 0;
+  EnsParams P{};
+  int bin_max = 0;      // largest bin (slot count index) a chain of this ensemble can need
+  // sampling
+  int ncomp = 0;        // floats per sample per chain (3 G(t), 8 flow)
+  int window = 0;
+  int64_t k_cur = 0;    // next sample index every healthy chain will record
+  int64_t stage_floats = 0;
+  int64_t last_k0 = 0, last_rows = 0;
+  // correlator (G(t))
+  int p = 0, m = 0, s = 0;
+  double* dD = nullptr;
+  double* dA = nullptr;
+  double* dC = nullptr;
+  std::vector<int> ins, macc;
+  std::vector<int64_t> nins;
+  // flow output rows (9 doubles per row)
+  std::vector<double> flow_rows;
+};
+
+}  // namespace
+
+struct dsm_engine {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int n_cu = 0;
+  // chains
+  int nch = 0;
+  int64_t nstr = 0;
+  ChainArrays ch{};
+  // ensembles
+  std::vector<Ens> ens;
+  EnsParams* d_ens = nullptr;
+  int d_ens_cap = 0;
+  std::vector<float> tabs;
+  float* d_tabs = nullptr;
+  size_t d_tabs_cap = 0;
+  std::vector<int64_t> stage_off;  // per ensemble
+  int64_t stage_used = 0;
+  float* d_stage = nullptr;
+  int64_t d_stage_cap = 0;
+  // work lists: [bin][nch] + counts
+  int* d_work = nullptr;
+  int d_work_cap = 0;
+  int* d_cnt = nullptr;     // [kNumBins] list counts, [kNumBins..2k) queues
+  unsigned long long* d_counters = nullptr;
+  int* d_corr_cnt = nullptr;
+  int d_corr_cnt_cap = 0;
+  double* d_red = nullptr;
+  int d_red_cap = 0;
+  int* d_healthy = nullptr;
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<int> host_work;
+};
+
+namespace {
+
+int set_device(dsm_engine* e) {
+  HIPCHK(hipSetDevice(e->device));
+  return DSM_OK;
+}
+
+// grow a device array of n_old elements to n_new, preserving contents
+template <typename T>
+int grow(T** p, int64_t n_old, int64_t n_new, hipStream_t st) {
+  T* q = nullptr;
+  int rc = dev_alloc(&q, (size_t)n_new);
+  if (rc) return rc;
+  if (*p && n_old > 0) HIPCHK(hipMemcpyAsync(q, *p, (size_t)n_old * sizeof(T), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (*p) HIPCHK(hipFree(*p));
+  *p = q;
+  return DSM_OK;
+}
+
+int grow_chains(dsm_engine* e, int n_new, int64_t s_new) {
+  ChainArrays& c = e->ch;
+  const int n0 = e->nch;
+  const int64_t s0 = e->nstr;
+  int rc = 0;
+  if ((rc = grow(&c.Z, n0, n_new, e->st)) || (rc = grow(&c.nk, n0, n_new, e->st)) ||
+      (rc = grow(&c.ens, n0, n_new, e->st)) || (rc = grow(&c.cap, n0, n_new, e->st)) ||
+      (rc = grow(&c.stall, n0, n_new, e->st)) || (rc = grow(&c.cid, n0, n_new, e->st)) ||
+      (rc = grow(&c.local, n0, n_new, e->st)) || (rc = grow(&c.base, n0, n_new, e->st)) ||
+      (rc = grow(&c.t, n0, n_new, e->st)) || (rc = grow(&c.ev, n0, n_new, e->st)) ||
+      (rc = grow(&c.dt_prev, n0, n_new, e->st)) || (rc = grow(&c.next_sample, n0, n_new, e->st)) ||
+      (rc = grow(&c.Qx, s0, s_new, e->st)) || (rc = grow(&c.Qy, s0, s_new, e->st)) ||
+      (rc = grow(&c.Qz, s0, s_new, e->st)) || (rc = grow(&c.N, s0, s_new, e->st)) ||
+      (rc = grow(&c.w, s0, s_new, e->st)))
+    return rc;
+  if (n_new > e->d_work_cap) {
+    if (e->d_work) HIPCHK(hipFree(e->d_work));
+    if ((rc = dev_alloc(&e->d_work, (size_t)kNumBins * n_new))) return rc;
+    e->d_work_cap = n_new;
+  }
+  e->nch = n_new;
+  e->nstr = s_new;
+  return DSM_OK;
+}
+
+int upload_ens(dsm_engine* e) {
+  const int n = (int)e->ens.size();
+  if (n > e->d_ens_cap) {
+    if (e->d_ens) HIPCHK(hipFree(e->d_ens));
+    int rc = dev_alloc(&e->d_ens, (size_t)std::max(n, 16));
+    if (rc) return rc;
+    e->d_ens_cap = std::max(n, 16);
+  }
+  std::vector<EnsParams> h(n);
+  for (int i = 0; i < n; ++i) h[i] = e->ens[i].P;
+  HIPCHK(hipMemcpyAsync(e->d_ens, h.data(), sizeof(EnsParams) * n, hipMemcpyHostToDevice, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  return DSM_OK;
+}
+
+int upload_tabs(dsm_engine* e) {
+  if (e->tabs.size() > e->d_tabs_cap) {
+    if (e->d_tabs) HIPCHK(hipFree(e->d_tabs));
+    const size_t cap = std::max(e->tabs.size() * 2, (size_t)4096);
+    int rc = dev_alloc(&e->d_tabs, cap);
+    if (rc) return rc;
+    e->d_tabs_cap = cap;
+  }
+  HIPCHK(hipMemcpy(e->d_tabs, e->tabs.data(), sizeof(float) * e->tabs.size(), hipMemcpyHostToDevice));
+  return DSM_OK;
+}
+
+int ensure_stage(dsm_engine* e) {
+  if (e->stage_used > e->d_stage_cap) {
+    if (e->d_stage) HIPCHK(hipFree(e->d_stage));
+    int rc = dev_alloc(&e->d_stage, (size_t)e->stage_used);
+    if (rc) return rc;
+    e->d_stage_cap = e->stage_used;
+  }
+  return DSM_OK;
+}
+
+Ens* get_ens(dsm_engine* e, int id) {
+  if (!e || id < 0 || id >= (int)e->ens.size() || !e->ens[id].live) return nullptr;
+  return &e->ens[id];
+}
+
+// correlator memory, sized when the level count is known
+int corr_alloc(dsm_engine* e, Ens& E, int s) {
+  if (E.s == s && E.dD) return DSM_OK;
+  if (E.dD) { (void)hipFree(E.dD); (void)hipFree(E.dA); (void)hipFree(E.dC); E.dD = E.dA = E.dC = nullptr; }
+  E.s = s;
+  const size_t n = (size_t)E.n;
+  int rc;
+  if ((rc = dev_alloc(&E.dD, (size_t)s * E.p * 3 * n)) || (rc = dev_alloc(&E.dA, (size_t)s * 3 * n)) ||
+      (rc = dev_alloc(&E.dC, (size_t)s * E.p * n)))
+    return rc;
+  HIPCHK(hipMemsetAsync(E.dD, 0, sizeof(double) * s * E.p * 3 * n, e->st));
+  HIPCHK(hipMemsetAsync(E.dA, 0, sizeof(double) * s * 3 * n, e->st));
+  HIPCHK(hipMemsetAsync(E.dC, 0, sizeof(double) * s * E.p * n, e->st));
+  E.ins.assign(s, 0);
+  E.macc.assign(s, 0);
+  E.nins.assign(s, 0);
+  if (3 * s > e->d_corr_cnt_cap) {
+    if (e->d_corr_cnt) HIPCHK(hipFree(e->d_corr_cnt));
+    if ((rc = dev_alloc(&e->d_corr_cnt, (size_t)3 * s))) return rc;
+    e->d_corr_cnt_cap = 3 * s;
+  }
+  if (s * E.p > e->d_red_cap) {
+    if (e->d_red) HIPCHK(hipFree(e->d_red));
+    if ((rc = dev_alloc(&e->d_red, (size_t)s * E.p))) return rc;
+    e->d_red_cap = s * E.p;
+  }
+  HIPCHK(hipStreamSynchronize(e->st));
+  return DSM_OK;
+}
+
+// host copy of the shared correlator counters after `rows` level-0 inserts
+void corr_advance_counters(Ens& E, int64_t rows) {
+  for (int64_t r = 0; r < rows; ++r) {
+    int lvl = 0;
+    for (;;) {
+      if (lvl == E.s) break;
+      E.macc[lvl]++;
+      const bool push = E.macc[lvl] == E.m;
+      if (push) E.macc[lvl] = 0;
+      E.ins[lvl] = (E.ins[lvl] + 1 == E.p) ? 0 : E.ins[lvl] + 1;
+      E.nins[lvl]++;
+      if (!push) break;
+      ++lvl;
+    }
+  }
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+int device_blocks(dsm_engine* e, int S, bool plain) {
+  int nb = kmc_max_blocks_per_cu(S, plain);
+  if (nb <= 0) nb = 1;
+  return nb * e->n_cu;
+}
+
+KmcArgs base_args(dsm_engine* e) {
+  KmcArgs a{};
+  a.ch = e->ch;
+  a.ens = e->d_ens;
+  a.tabs = e->d_tabs;
+  a.stage = e->d_stage;
+  a.counters = e->d_counters;
+  a.max_events = INT64_MAX;
+  return a;
+}
+
+// One KMC pass over every chain of the given ensembles until each reaches its
+// ensemble's P.t_end (batched: persistent Z-binned launches; plain: one jump per
+// launch until no chain moves).  P must already be uploaded.
+int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, int64_t max_events,
+             volatile const int32_t* run_flag, dsm_run_stats* rs) {
+  // work list = all chains of the ensembles, bin 0 gets everything, kernels push overflow up
+  int total = 0, bin_max = 0;
+  for (int id : ids) { total += e->ens[id].n; bin_max = std::max(bin_max, e->ens[id].bin_max); }
+  if (total == 0) return DSM_OK;
+  int* cnt = e->d_cnt;
+  int* list0 = e->d_work;
+  int off = 0;
+  for (int id : ids) {
+    int rc = (int)launch_iota(list0 + off, e->ens[id].chain0, e->ens[id].n, e->st);
+    if (rc) return fail(DSM_ERR_DEVICE, "iota launch failed");
+    off += e->ens[id].n;
+  }
+  KmcArgs a = base_args(e);
+  a.max_events = max_events;
+  if (mode == DSM_MODE_PLAIN) {
+    // plain: every chain in the slot count of its ensemble's longest possible chain
+    const int S = kSlots[bin_max];
+    HIPCHK(launch_fill_u32(cnt, (uint32_t)total, 1, e->st));
+    a.work = list0;
+    a.n_work = cnt;
+    a.overflow = e->d_work + (size_t)e->d_work_cap;  // never used at the maximal slot count
+    a.n_overflow = cnt + kNumBins;
+    a.max_events = 1;
+    const int batch = 16;
+    for (;;) {
+      unsigned long long before = 0, after = 0;
+      HIPCHK(hipMemcpyAsync(&before, e->d_counters, sizeof(before), hipMemcpyDeviceToHost, e->st));
+      HIPCHK(hipEventRecord(e->evs[0], e->st));
+      for (int b = 0; b < batch; ++b) {
+        HIPCHK(launch_kmc(a, S, true, total, e->st));
+        rs->kmc_launches++;
+        rs->first_launches++;
+      }
+      HIPCHK(hipEventRecord(e->evs[1], e->st));
+      HIPCHK(hipMemcpyAsync(&after, e->d_counters, sizeof(after), hipMemcpyDeviceToHost, e->st));
+      HIPCHK(hipStreamSynchronize(e->st));
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+      rs->kmc_ms += ms;
+      rs->first_kmc_ms += ms;
+      if (after == before) break;
+      if (run_flag && *run_flag == 0) return DSM_ABORTED;
+    }
+    return DSM_OK;
+  }
+  // batched
+  HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumBins, e->st));
+  HIPCHK(launch_fill_u32(cnt, (uint32_t)total, 1, e->st));
+  for (int b = 0; b <= bin_max; ++b) {
+    a.work = e->d_work + (size_t)b * e->d_work_cap;
+    a.n_work = cnt + b;
+    a.queue = cnt + kNumBins + b;
+    const int nb = std::min(b + 1, kNumBins - 1);
+    a.overflow = e->d_work + (size_t)nb * e->d_work_cap;
+    a.n_overflow = cnt + nb;
+    const int S = kSlots[b];
+    const int grid = device_blocks(e, S, false);
+    HIPCHK(hipEventRecord(e->evs[0], e->st));
+    HIPCHK(launch_kmc(a, S, false, grid, e->st));
+    HIPCHK(hipEventRecord(e->evs[1], e->st));
+    HIPCHK(hipEventSynchronize(e->evs[1]));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+    rs->kmc_ms += ms;
+    rs->kmc_launches++;
+    if (b == 0) { rs->first_kmc_ms += ms; rs->first_launches++; }
+  }
+  if (run_flag && *run_flag == 0) return DSM_ABORTED;
+  return DSM_OK;
+}
+
+int finish_stats(dsm_engine* e, dsm_run_stats* rs, const Timer& T) {
+  unsigned long long c[2];
+  HIPCHK(hipMemcpy(c, e->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+  rs->events = c[0];
+  rs->z_sum = c[1];
+  rs->total_ms = T.ms();
+  return DSM_OK;
+}
+
+int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<double>& t_end, int mode,
+                 volatile const int32_t* run_flag, volatile int32_t* progress, dsm_run_stats* out) {
+  Timer T;
+  dsm_run_stats rs{};
+  int rc = set_device(e);
+  if (rc) return rc;
+  HIPCHK(launch_fill_u32(e->d_counters, 0u, 4, e->st));
+  // per-ensemble sample targets
+  std::vector<int64_t> k_target(ids.size(), 0);
+  for (size_t q = 0; q < ids.size(); ++q) {
+    Ens& E = e->ens[ids[q]];
+    const double te = t_end[q];
+    if (E.P.sample_mode == kSampleEq) {
+      k_target[q] = (int64_t)std::ceil(te / E.P.delta - 1e-12);
+      if (E.s == 0) {
+        int s = E.d.corr_levels;
+        if (s <= 0) s = (int)std::ceil(std::log(te / E.P.delta / E.p) / std::log((double)E.m)) + 1;
+        if (s < 1) s = 1;
+        if (s > 40) return fail(DSM_ERR_ARG, "correlator needs %d levels (max 40)", s);
+        if ((rc = corr_alloc(e, E, s))) return rc;
+      }
+    } else if (E.P.sample_mode == kSampleFlow) {
+      k_target[q] = (int64_t)std::ceil(te / E.P.delta - 1e-12) + 1;
+    }
+  }
+  const double t0_prog = 0.0;
+  (void)t0_prog;
+  for (;;) {
+    // next window: each ensemble advances to min(window end, t_end)
+    bool any = false;
+    std::vector<int> act;
+    for (size_t q = 0; q < ids.size(); ++q) {
+      Ens& E = e->ens[ids[q]];
+      if (E.P.sample_mode == kSampleNone) {
+        if (E.P.t_end != t_end[q] || !any) {}
+        E.P.t_end = t_end[q];
+        E.P.k0 = E.k_cur;
+        E.P.k_end = E.k_cur;
+        act.push_back(ids[q]);
+        continue;
+      }
+      if (E.k_cur >= k_target[q]) {
+        // samples done; still bring the clock to t_end
+        E.P.k0 = E.k_cur;
+        E.P.k_end = E.k_cur;
+        E.P.t_end = t_end[q];
+        act.push_back(ids[q]);
+        continue;
+      }
+      const int64_t k_end = std::min(E.k_cur + E.window, k_target[q]);
+      E.P.k0 = E.k_cur;
+      E.P.k_end = k_end;
+      if (E.P.sample_mode == kSampleEq) E.P.t_end = std::min((double)k_end * E.P.delta, t_end[q]);
+      else E.P.t_end = std::min((double)(k_end - 1) * E.P.delta, t_end[q]);
+      act.push_back(ids[q]);
+      any = true;
+    }
+    if ((rc = upload_ens(e))) return rc;
+    for (int id : act) {
+      Ens& E = e->ens[id];
+      if (E.P.k_end > E.P.k0)
+        HIPCHK(launch_fill_u32(e->d_stage + E.P.stage_off, 0u, (E.P.k_end - E.P.k0) * E.n * E.ncomp, e->st));
+    }
+    rc = kmc_pass(e, act, mode, INT64_MAX, run_flag, &rs);
+    if (rc == DSM_ABORTED) { finish_stats(e, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
+    if (rc) return rc;
+    // consume the window's samples
+    for (int id : act) {
+      Ens& E = e->ens[id];
+      const int64_t rows = E.P.k_end - E.P.k0;
+      if (rows <= 0) continue;
+      E.last_k0 = E.P.k0;
+      E.last_rows = rows;
+      if (E.P.sample_mode == kSampleEq) {
+        std::vector<int> cnt(3 * E.s);
+        for (int k = 0; k < E.s; ++k) {
+          cnt[k] = E.ins[k];
+          cnt[E.s + k] = E.macc[k];
+          cnt[2 * E.s + k] = (int)std::min<int64_t>(E.nins[k], E.p);
+        }
+        HIPCHK(hipMemcpyAsync(e->d_corr_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, e->st));
+        HIPCHK(launch_corr_update(e->d_stage + E.P.stage_off, E.dD, E.dA, E.dC, e->ch.stall, E.chain0, E.n, E.p, E.m,
+                                  E.s, (int)rows, e->d_corr_cnt, e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
+        corr_advance_counters(E, rows);
+      } else {
+        double* d_rows = nullptr;
+        if ((rc = dev_alloc(&d_rows, (size_t)rows * 9))) return rc;
+        HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, e->ch.stall, E.chain0, E.n, (int)rows, d_rows, e->st));
+        const size_t old = E.flow_rows.size();
+        E.flow_rows.resize(old + (size_t)rows * 9);
+        HIPCHK(hipMemcpyAsync(E.flow_rows.data() + old, d_rows, sizeof(double) * rows * 9, hipMemcpyDeviceToHost,
+                              e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
+        HIPCHK(hipFree(d_rows));
+      }
+      E.k_cur = E.P.k_end;
+    }
+    if (progress && !ids.empty()) {
+      Ens& E = e->ens[ids[0]];
+      const double frac = k_target[0] > 0 ? (double)E.k_cur / (double)k_target[0] : 1.0;
+      *progress = (int32_t)(100.0 * frac);
+    }
+    if (run_flag && *run_flag == 0) { finish_stats(e, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
+    if (!any) break;
+  }
+  if ((rc = finish_stats(e, &rs, T))) return rc;
+  if (out) *out = rs;
+  return DSM_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" {
+
+const char* dsm_last_error(void) { return g_err.c_str(); }
+
+int dsm_mem_info(int32_t device_id, size_t* free_bytes, size_t* total_bytes) {
+  HIPCHK(hipSetDevice(device_id));
+  size_t f = 0, t = 0;
+  HIPCHK(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return DSM_OK;
+}
+
+int dsm_create(int32_t device_id, dsm_engine** out) {
+  if (!out) return fail(DSM_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(DSM_ERR_DEVICE, "no HIP device found");
+  if (device_id < 0 || device_id >= ndev) return fail(DSM_ERR_DEVICE, "device %d out of range (%d devices)", device_id, ndev);
+  HIPCHK(hipSetDevice(device_id));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device_id));
+  dsm_engine* e = new dsm_engine();
+  e->device = device_id;
+  e->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail(DSM_ERR_DEVICE, "stream creation failed");
+  }
+  for (auto& ev : e->evs) HIPCHK(hipEventCreate(&ev));
+  int rc;
+  if ((rc = dev_alloc(&e->d_cnt, 4 * kNumBins)) || (rc = dev_alloc(&e->d_counters, 4)) ||
+      (rc = dev_alloc(&e->d_healthy, 1))) {
+    dsm_destroy(e);
+    return rc;
+  }
+  // verify the code object runs on this device
+  hipError_t le = launch_fill_u32(e->d_counters, 0u, 8, e->st);
+  if (le != hipSuccess || hipStreamSynchronize(e->st) != hipSuccess) {
+    dsm_destroy(e);
+    return fail(DSM_ERR_NOKERNEL, "gfx950 code object did not launch on device %d (%s)", device_id, prop.gcnArchName);
+  }
+  *out = e;
+  return DSM_OK;
+}
+
+int dsm_destroy(dsm_engine* e) {
+  if (!e) return DSM_OK;
+  (void)hipSetDevice(e->device);
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  for (auto& E : e->ens) {
+    if (E.dD) (void)hipFree(E.dD);
+    if (E.dA) (void)hipFree(E.dA);
+    if (E.dC) (void)hipFree(E.dC);
+  }
+  ChainArrays& c = e->ch;
+  void* ptrs[] = {c.Z, c.nk, c.ens, c.cap, c.stall, c.cid, c.local, c.base, c.t, c.ev, c.dt_prev, c.next_sample,
+                  c.Qx, c.Qy, c.Qz, c.N, c.w, e->d_ens, e->d_tabs, e->d_stage, e->d_work, e->d_cnt, e->d_counters,
+                  e->d_corr_cnt, e->d_red, e->d_healthy};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& ev : e->evs)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  delete e;
+  return DSM_OK;
+}
+
+int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id) {
+  if (!e || !d) return fail(DSM_ERR_ARG, "null argument");
+  if (d->n_chains <= 0) return fail(DSM_ERR_ARG, "n_chains must be positive");
+  if (!(d->beta > 0.0f)) return fail(DSM_ERR_ARG, "beta must be positive");
+  if ((d->sample_mode != DSM_SAMPLE_NONE) && !(d->delta > 0.0)) return fail(DSM_ERR_ARG, "delta must be positive");
+  int rc = set_device(e);
+  if (rc) return rc;
+  Ens E;
+  E.d = *d;
+  E.d.nk_per_chain = nullptr;
+  E.n = d->n_chains;
+  E.nk.resize(E.n);
+  int nmax = 0;
+  for (int i = 0; i < E.n; ++i) {
+    const int nk = d->nk_per_chain ? d->nk_per_chain[i] : d->nk;
+    if (nk < 2 || nk > 1023) return fail(DSM_ERR_ARG, "Nk=%d outside 2..1023", nk);
+    E.nk[i] = nk;
+    nmax = std::max(nmax, nk);
+  }
+  E.nmax = nmax;
+  E.bin_max = slots_for(nmax);
+  if (E.bin_max < 0) return fail(DSM_ERR_ARG, "Nk too large");
+  // parameters
+  const int nk_ref = d->nk_per_chain ? nmax : d->nk;
+  Pcd pc;
+  pcd_init(pc, d->beta, (float)nk_ref);
+  EnsParams& P = E.P;
+  P.beta = d->beta;
+  P.nk = nk_ref;
+  P.nmax = nmax;
+  P.cd = d->cd ? 1 : 0;
+  P.flow = 0;
+  for (int i = 0; i < 9; ++i) {
+    P.kappa[i] = d->kappa[i];
+    if (d->kappa[i] != 0.0f) P.flow = 1;
+  }
+  P.exp_wait = d->exp_wait ? 1 : 0;
+  P.sample_mode = d->sample_mode;
+  P.nchains = E.n;
+  P.c_create = pc.normdt / d->beta;
+  P.s_At = pc.s_At; P.s_Ct = pc.s_Ct; P.s_Dt = pc.s_Dt; P.s_Adt = pc.s_Adt; P.s_Bdt = pc.s_Bdt;
+  P.s_Cdt = pc.s_Cdt; P.s_Ddt = pc.s_Ddt; P.s_taud_inv = pc.s_taud_inv; P.s_one_minus_g = pc.s_one_minus_g;
+  {
+    const double pcut = 1.0 / (1.0 + (double)d->beta);
+    double Tc = std::floor(pcut * 4294967296.0);
+    if (Tc > 4294967295.0) Tc = 4294967295.0;
+    P.cut_threshold = (uint32_t)Tc;
+  }
+  P.key0 = d->seed;
+  P.key1 = d->salt;
+  P.delta = d->delta;
+  P.t_end = 0.0;
+  P.k0 = 0;
+  P.k_end = 0;
+  // tables
+  std::vector<float> tb;
+  build_tables(tb, nmax, d->beta, P.cd, P.c_create);
+  P.tab_off = (int)e->tabs.size();
+  P.fr_off = P.tab_off + kNumTables * (nmax + 2);
+  e->tabs.insert(e->tabs.end(), tb.begin(), tb.end());
+  if ((rc = upload_tabs(e))) return rc;
+  // sampling
+  E.ncomp = d->sample_mode == DSM_SAMPLE_GT ? 3 : d->sample_mode == DSM_SAMPLE_FLOW ? 8 : 0;
+  E.window = d->window > 0 ? d->window : kDefaultWindow;
+  E.k_cur = d->sample_mode == DSM_SAMPLE_FLOW ? 1 : 0;
+  E.p = d->corr_p > 0 ? d->corr_p : 64;
+  E.m = d->corr_m > 0 ? d->corr_m : 8;
+  if (E.p % E.m != 0 && d->sample_mode == DSM_SAMPLE_GT) return fail(DSM_ERR_ARG, "corr_p must be a multiple of corr_m");
+  E.stage_floats = (int64_t)E.window * E.n * E.ncomp;
+  P.stage_off = e->stage_used;
+  e->stage_used += E.stage_floats;
+  if ((rc = ensure_stage(e))) return rc;
+  // chains
+  E.chain0 = e->nch;
+  E.strand0 = e->nstr;
+  E.nstrand = 0;
+  for (int i = 0; i < E.n; ++i) E.nstrand += E.nk[i];
+  P.chain0 = E.chain0;
+  if ((rc = grow_chains(e, e->nch + E.n, e->nstr + E.nstrand))) return rc;
+  {
+    const int id = (int)e->ens.size();
+    std::vector<int> hnk(E.n), hens(E.n, id), hcap(E.n), hzero(E.n, 0), hloc(E.n);
+    std::vector<uint32_t> hcid(E.n);
+    std::vector<int64_t> hbase(E.n);
+    int64_t b = E.strand0;
+    for (int i = 0; i < E.n; ++i) {
+      hnk[i] = E.nk[i]; hcap[i] = E.nk[i]; hcid[i] = (uint32_t)i; hloc[i] = i; hbase[i] = b; b += E.nk[i];
+    }
+    const int c0 = E.chain0;
+    HIPCHK(hipMemcpy(e->ch.nk + c0, hnk.data(), 4 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ch.ens + c0, hens.data(), 4 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ch.cap + c0, hcap.data(), 4 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ch.cid + c0, hcid.data(), 4 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ch.local + c0, hloc.data(), 4 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ch.base + c0, hbase.data(), 8 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ch.stall + c0, hzero.data(), 4 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ch.Z + c0, hzero.data(), 4 * E.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(e->ch.t + c0, 0, 8 * E.n));
+    HIPCHK(hipMemset(e->ch.ev + c0, 0, 8 * E.n));
+    HIPCHK(hipMemset(e->ch.dt_prev + c0, 0, 4 * E.n));
+    std::vector<int64_t> hns(E.n, E.k_cur);
+    HIPCHK(hipMemcpy(e->ch.next_sample + c0, hns.data(), 8 * E.n, hipMemcpyHostToDevice));
+    E.live = true;
+    e->ens.push_back(std::move(E));
+    if ((rc = upload_ens(e))) return rc;
+    if (ens_id) *ens_id = id;
+  }
+  return DSM_OK;
+}
+
+int dsm_remove_ensemble(dsm_engine* e, int32_t ens) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  (void)set_device(e);
+  if (E->dD) { (void)hipFree(E->dD); (void)hipFree(E->dA); (void)hipFree(E->dC); E->dD = E->dA = E->dC = nullptr; }
+  E->live = false;
+  E->flow_rows.clear();
+  return DSM_OK;
+}
+
+int dsm_init_equilibrium(dsm_engine* e, int32_t ens) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  int rc = set_device(e);
+  if (rc) return rc;
+  HIPCHK(launch_iota(e->d_work, E->chain0, E->n, e->st));
+  HIPCHK(launch_init_eq(e->ch, e->d_ens, e->d_work, E->n, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  E->k_cur = E->P.sample_mode == kSampleFlow ? 1 : 0;
+  E->flow_rows.clear();
+  return DSM_OK;
+}
+
+int dsm_state_cap(dsm_engine* e, int32_t ens, int32_t* cap) {
+  Ens* E = get_ens(e, ens);
+  if (!E || !cap) return fail(DSM_ERR_ARG, "bad ensemble or pointer");
+  *cap = E->nmax;
+  return DSM_OK;
+}
+
+int dsm_get_state(dsm_engine* e, int32_t ens, int32_t cap, int32_t* Z, int32_t* N, float* Q, float* w, double* t,
+                  uint64_t* ev, int32_t* stall, float* dt_prev, int64_t* next_sample) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  if (cap < E->nmax) return fail(DSM_ERR_ARG, "cap %d < Nk max %d", cap, E->nmax);
+  int rc = set_device(e);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(e->st));
+  const int n = E->n, c0 = E->chain0;
+  std::vector<int> hZ(n);
+  HIPCHK(hipMemcpy(hZ.data(), e->ch.Z + c0, 4 * n, hipMemcpyDeviceToHost));
+  if (Z) std::memcpy(Z, hZ.data(), 4 * n);
+  if (t) HIPCHK(hipMemcpy(t, e->ch.t + c0, 8 * n, hipMemcpyDeviceToHost));
+  if (ev) HIPCHK(hipMemcpy(ev, e->ch.ev + c0, 8 * n, hipMemcpyDeviceToHost));
+  if (stall) HIPCHK(hipMemcpy(stall, e->ch.stall + c0, 4 * n, hipMemcpyDeviceToHost));
+  if (dt_prev) HIPCHK(hipMemcpy(dt_prev, e->ch.dt_prev + c0, 4 * n, hipMemcpyDeviceToHost));
+  if (next_sample) HIPCHK(hipMemcpy(next_sample, e->ch.next_sample + c0, 8 * n, hipMemcpyDeviceToHost));
+  if (N || Q || w) {
+    const int64_t ns = E->nstrand, s0 = E->strand0;
+    std::vector<float> qx(ns), qy(ns), qz(ns), ww(ns);
+    std::vector<int> nn(ns);
+    HIPCHK(hipMemcpy(qx.data(), e->ch.Qx + s0, 4 * ns, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(qy.data(), e->ch.Qy + s0, 4 * ns, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(qz.data(), e->ch.Qz + s0, 4 * ns, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ww.data(), e->ch.w + s0, 4 * ns, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nn.data(), e->ch.N + s0, 4 * ns, hipMemcpyDeviceToHost));
+    int64_t b = 0;
+    for (int i = 0; i < n; ++i) {
+      for (int k = 0; k < cap; ++k) {
+        const bool on = k < hZ[i];
+        const size_t o = (size_t)i * cap + k;
+        if (N) N[o] = on ? nn[b + k] : 0;
+        if (Q) {
+          Q[o * 3 + 0] = on ? qx[b + k] : 0.0f;
+          Q[o * 3 + 1] = on ? qy[b + k] : 0.0f;
+          Q[o * 3 + 2] = on ? qz[b + k] : 0.0f;
+        }
+        if (w) w[o] = (k < hZ[i] - 1) ? ww[b + k] : 0.0f;
+      }
+      b += E->nk[i];
+    }
+  }
+  return DSM_OK;
+}
+
+int dsm_set_state(dsm_engine* e, int32_t ens, int32_t cap, const int32_t* Z, const int32_t* N, const float* Q,
+                  const float* w, const double* t, const uint64_t* ev, const int32_t* stall, const float* dt_prev,
+                  const int64_t* next_sample) {
+  Ens* E = get_ens(e, ens);
+  if (!E || !Z || !N || !Q || !w) return fail(DSM_ERR_ARG, "bad ensemble or null state array");
+  if (cap < E->nmax) return fail(DSM_ERR_ARG, "cap %d < Nk max %d", cap, E->nmax);
+  const int n = E->n, c0 = E->chain0;
+  for (int i = 0; i < n; ++i) {
+    if (Z[i] < 1 || Z[i] > E->nk[i]) return fail(DSM_ERR_ARG, "chain %d: Z=%d outside 1..%d", i, Z[i], E->nk[i]);
+    int64_t sum = 0;
+    for (int k = 0; k < Z[i]; ++k) {
+      const int v = N[(size_t)i * cap + k];
+      if (v < 1) return fail(DSM_ERR_ARG, "chain %d strand %d: N=%d < 1", i, k, v);
+      sum += v;
+    }
+    if (sum != E->nk[i]) return fail(DSM_ERR_ARG, "chain %d: sum N = %lld != Nk %d", i, (long long)sum, E->nk[i]);
+  }
+  int rc = set_device(e);
+  if (rc) return rc;
+  const int64_t ns = E->nstrand, s0 = E->strand0;
+  std::vector<float> qx(ns, 0.0f), qy(ns, 0.0f), qz(ns, 0.0f), ww(ns, 0.0f);
+  std::vector<int> nn(ns, 0);
+  int64_t b = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < Z[i]; ++k) {
+      const size_t o = (size_t)i * cap + k;
+      nn[b + k] = N[o]; qx[b + k] = Q[o * 3 + 0]; qy[b + k] = Q[o * 3 + 1]; qz[b + k] = Q[o * 3 + 2];
+      ww[b + k] = (k < Z[i] - 1) ? w[o] : 0.0f;
+    }
+    b += E->nk[i];
+  }
+  HIPCHK(hipStreamSynchronize(e->st));
+  HIPCHK(hipMemcpy(e->ch.Qx + s0, qx.data(), 4 * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.Qy + s0, qy.data(), 4 * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.Qz + s0, qz.data(), 4 * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.w + s0, ww.data(), 4 * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.N + s0, nn.data(), 4 * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.Z + c0, Z, 4 * n, hipMemcpyHostToDevice));
+  std::vector<double> zt(n, 0.0);
+  std::vector<uint64_t> zev(n, 0);
+  std::vector<int> zst(n, 0);
+  std::vector<float> zdt(n, 0.0f);
+  std::vector<int64_t> zns(n, E->k_cur);
+  HIPCHK(hipMemcpy(e->ch.t + c0, t ? t : zt.data(), 8 * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.ev + c0, ev ? ev : zev.data(), 8 * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.stall + c0, stall ? stall : zst.data(), 4 * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.dt_prev + c0, dt_prev ? dt_prev : zdt.data(), 4 * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->ch.next_sample + c0, next_sample ? next_sample : zns.data(), 8 * n, hipMemcpyHostToDevice));
+  if (next_sample) E->k_cur = next_sample[0];
+  return DSM_OK;
+}
+
+int dsm_advance(dsm_engine* e, int32_t ens, double t_end, int32_t mode, volatile const int32_t* run_flag,
+                volatile int32_t* progress, dsm_run_stats* stats) {
+  if (!get_ens(e, ens)) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  if (mode != DSM_MODE_BATCHED && mode != DSM_MODE_PLAIN) return fail(DSM_ERR_ARG, "bad mode %d", mode);
+  return advance_impl(e, {ens}, {t_end}, mode, run_flag, progress, stats);
+}
+
+int dsm_advance_many(dsm_engine* e, const int32_t* ens_ids, const double* t_end, int32_t count, int32_t mode,
+                     volatile const int32_t* run_flag, dsm_run_stats* stats) {
+  if (!e || !ens_ids || !t_end || count <= 0) return fail(DSM_ERR_ARG, "bad arguments");
+  std::vector<int> ids(ens_ids, ens_ids + count);
+  std::vector<double> te(t_end, t_end + count);
+  for (int id : ids)
+    if (!get_ens(e, id)) return fail(DSM_ERR_ARG, "no ensemble %d", id);
+  return advance_impl(e, ids, te, mode, run_flag, nullptr, stats);
+}
+
+int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mode, dsm_run_stats* stats) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  if (n_events < 0) return fail(DSM_ERR_ARG, "negative event count");
+  int rc = set_device(e);
+  if (rc) return rc;
+  Timer T;
+  dsm_run_stats rs{};
+  HIPCHK(launch_fill_u32(e->d_counters, 0u, 4, e->st));
+  E->P.t_end = INFINITY;
+  E->P.k0 = E->k_cur;
+  E->P.k_end = E->k_cur;  // sampling suspended
+  if ((rc = upload_ens(e))) return rc;
+  if (mode == DSM_MODE_PLAIN) {
+    const int S = kSlots[E->bin_max];
+    HIPCHK(launch_iota(e->d_work, E->chain0, E->n, e->st));
+    HIPCHK(launch_fill_u32(e->d_cnt, (uint32_t)E->n, 1, e->st));
+    KmcArgs a = base_args(e);
+    a.work = e->d_work;
+    a.n_work = e->d_cnt;
+    a.overflow = e->d_work + (size_t)e->d_work_cap;
+    a.n_overflow = e->d_cnt + kNumBins;
+    a.max_events = 1;
+    HIPCHK(hipEventRecord(e->evs[0], e->st));
+    for (int64_t k = 0; k < n_events; ++k) {
+      HIPCHK(launch_kmc(a, S, true, E->n, e->st));
+      rs.kmc_launches++;
+    }
+    HIPCHK(hipEventRecord(e->evs[1], e->st));
+    HIPCHK(hipEventSynchronize(e->evs[1]));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+    rs.kmc_ms = ms;
+    rs.first_kmc_ms = ms;
+    rs.first_launches = rs.kmc_launches;
+  } else {
+    if ((rc = kmc_pass(e, {ens}, mode, n_events, nullptr, &rs))) return rc;
+  }
+  if ((rc = finish_stats(e, &rs, T))) return rc;
+  if (stats) *stats = rs;
+  return DSM_OK;
+}
+
+int dsm_flow_stress(dsm_engine* e, int32_t ens, int64_t first_row, int64_t max_rows, double* out, int64_t* n_rows) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  const int64_t have = (int64_t)E->flow_rows.size() / 9;
+  if (n_rows) *n_rows = have;
+  if (out && first_row < have) {
+    const int64_t k = std::min(max_rows, have - first_row);
+    std::memcpy(out, E->flow_rows.data() + first_row * 9, sizeof(double) * 9 * k);
+  }
+  return DSM_OK;
+}
+
+int dsm_correlator(dsm_engine* e, int32_t ens, int32_t max_pts, double* lag, double* g, int32_t* npts) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  if (E->P.sample_mode != kSampleEq || !E->dC) return fail(DSM_ERR_STATE, "ensemble %d has no G(t) correlator", ens);
+  int rc = set_device(e);
+  if (rc) return rc;
+  const int s = E->s, p = E->p, m = E->m;
+  HIPCHK(launch_corr_reduce(E->dC, e->ch.stall, E->chain0, E->n, s * p, e->d_red, e->d_healthy, e->st));
+  std::vector<double> sums(s * p);
+  int healthy = 0;
+  HIPCHK(hipMemcpyAsync(sums.data(), e->d_red, sizeof(double) * s * p, hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipMemcpyAsync(&healthy, e->d_healthy, sizeof(int), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  int im = 0;
+  double mk = 1.0;
+  for (int k = 0; k < s; ++k) {
+    const int jmin = k == 0 ? 0 : p / m;
+    for (int j = jmin; j < p; ++j) {
+      const double npair = (double)std::max<int64_t>(0, E->nins[k] - j);
+      if (im < max_pts) {
+        if (lag) lag[im] = (double)j * mk;
+        if (g) g[im] = (npair > 0 && healthy > 0) ? sums[k * p + j] / npair / 3.0 / healthy : 0.0;
+      }
+      ++im;
+    }
+    mk *= m;
+  }
+  if (npts) *npts = im;
+  return DSM_OK;
+}
+
+int dsm_last_samples(dsm_engine* e, int32_t ens, float* out, int64_t max_floats, int64_t* k0, int64_t* nrows) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  const int64_t nf = E->last_rows * E->n * E->ncomp;
+  if (k0) *k0 = E->last_k0;
+  if (nrows) *nrows = E->last_rows;
+  if (out && nf > 0) {
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    HIPCHK(hipMemcpy(out, e->d_stage + E->P.stage_off, sizeof(float) * std::min(nf, max_floats),
+                     hipMemcpyDeviceToHost));
+  }
+  return DSM_OK;
+}
+
+int dsm_tables(dsm_engine* e, int32_t ens, int32_t which, float* out, int32_t* n) {
+  Ens* E = get_ens(e, ens);
+  if (!E || which < 0 || which > 9) return fail(DSM_ERR_ARG, "bad ensemble or table");
+  const int nn = E->nmax + 2;
+  const int cnt = which == 9 ? 4 * E->nmax + 4 : nn;
+  const int off = which == 9 ? E->P.fr_off : E->P.tab_off + which * nn;
+  if (n) *n = cnt;
+  if (out) std::memcpy(out, e->tabs.data() + off, sizeof(float) * cnt);
+  return DSM_OK;
+}
+
+int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* o) {
+  Ens* E = get_ens(e, ens);
+  if (!E || !o) return fail(DSM_ERR_ARG, "bad ensemble or pointer");
+  Pcd pc;
+  pcd_init(pc, E->d.beta, (float)E->P.nk);
+  const float v[24] = {pc.g, pc.alpha, pc.tau_0, pc.tau_max, pc.tau_d, pc.c1, pc.cm1, pc.rcm1c1,
+                       pc.At, pc.Adt, pc.Bdt, pc.normdt, pc.s_At, pc.s_Ct, pc.s_Dt, pc.s_Adt,
+                       pc.s_Bdt, pc.s_Cdt, pc.s_Ddt, pc.s_taud_inv, pc.s_one_minus_g, E->P.c_create,
+                       (float)E->P.cut_threshold, E->P.beta};
+  std::memcpy(o, v, sizeof(v));
+  return DSM_OK;
+}
+
+// ------------------------------------------------------------------ probes
+static int probe_run(int32_t device_id, const void* in, size_t in_bytes, void* out, size_t out_bytes,
+                     hipError_t (*launch)(const void*, void*, int, int, hipStream_t), int n, int which) {
+  HIPCHK(hipSetDevice(device_id));
+  void *din = nullptr, *dout = nullptr;
+  HIPCHK(hipMalloc(&din, in_bytes));
+  HIPCHK(hipMalloc(&dout, out_bytes));
+  HIPCHK(hipMemcpy(din, in, in_bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(dout, 0, out_bytes));
+  HIPCHK(launch(din, dout, n, which, nullptr));
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));
+  HIPCHK(hipFree(din));
+  HIPCHK(hipFree(dout));
+  return DSM_OK;
+}
+
+static hipError_t l_scan(const void* i, void* o, int n, int, hipStream_t s) {
+  return launch_scan_probe((const float*)i, (float*)o, n, s);
+}
+static hipError_t l_math(const void* i, void* o, int n, int w, hipStream_t s) {
+  return launch_math_probe((const float*)i, (float*)o, n, w, s);
+}
+static hipError_t l_philox(const void* i, void* o, int n, int, hipStream_t s) {
+  return launch_philox_probe((const uint32_t*)i, (uint32_t*)o, n, s);
+}
+
+int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n) {
+  if (n <= 0 || !in || !out) return fail(DSM_ERR_ARG, "bad probe arguments");
+  const int np = (n + 63) / 64 * 64;
+  std::vector<float> buf(np, 0.0f), res(np, 0.0f);
+  std::memcpy(buf.data(), in, 4 * n);
+  int rc = probe_run(device_id, buf.data(), 4 * np, res.data(), 4 * np, l_scan, n, 0);
+  if (rc) return rc;
+  std::memcpy(out, res.data(), 4 * n);
+  return DSM_OK;
+}
+int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which) {
+  if (n <= 0 || !in || !out || which < 0 || which > 3) return fail(DSM_ERR_ARG, "bad probe arguments");
+  return probe_run(device_id, in, 4 * (size_t)n, out, 4 * (size_t)n, l_math, n, which);
+}
+int dsm_probe_philox(int32_t device_id, const uint32_t* ck, uint32_t* out, int32_t n) {
+  if (n <= 0 || !ck || !out) return fail(DSM_ERR_ARG, "bad probe arguments");
+  return probe_run(device_id, ck, 24 * (size_t)n, out, 16 * (size_t)n, l_philox, n, 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,326 @@
+// dsm_kernels.hip -- MI355X (gfx950) kernels for the Discrete Slip-Link Model ensemble KMC path.
+//
+// One wave64 owns one chain.  Strand i sits in lane i%64 of register slot i/64
+// (S slots per lane, S a template parameter, so a chain of up to 64*S strands
+// lives in VGPRs).  A KMC jump (ktaletsk/gpu_dsm common/eq_ensemble_kernel.cu:242-663,
+// common/ensemble_kernel.cu:188-502) is:
+//   1. per-pair rates (sliding + constraint dynamics) cached per lane; the five
+//      chain-end rates summed into the last position,
+//   2. a wave64 inclusive scan (DPP tree) -> total W, dt = 1/W, FP64 clock += dt,
+//   3. pr = W*u from Philox(seed, chain, event); the pair is the first lane
+//      whose inclusive sum exceeds pr (ballot + ffs); the sub-event follows the
+//      reference order CDd, SD+, SD-, CDc (pairs) / CDc tail, SDc head/tail,
+//      SDd head/tail (ends),
+//   4. the effect: N updates for shifts; split/merge as in-wave shifts of all
+//      strand records; only the pair rates next to the change are recomputed.
+// Two forms share this code:
+//   * plain:   one jump per chain per launch, state read from and written to HBM
+//              (the reference's one-step-per-launch structure);
+//   * batched: persistent waves pull chains from a work queue, keep the chain in
+//              registers for many jumps (until its clock passes the window end),
+//              then write it back once.
+#include "dsm_kmc.h"
+
+namespace dsm {
+
+// ------------------------------------------------------------------ equilibrium init
+// Thread per chain.  Inner Kuhn-step boundary i (1 <= i < nk) carries an
+// entanglement iff Philox word (i-1) < 2^32/(1+beta): Z-1 ~ Binomial(nk-1, 1/(1+beta))
+// and N|Z uniform over compositions (the laws of common/chain.cu:38-88).
+// Internal strands get Q ~ N(0, N/3) per component (common/chain.cu:90-102),
+// links get CD rates from f_t (common/chain.cu:155-158).
+__global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsParams* ens, const int* chains,
+                                                        int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int c = chains[idx];
+  const EnsParams& P = ens[ch.ens[c]];
+  EnsLocal E;
+  load_ens(E, ens, ch.ens[c]);
+  const int nk = ch.nk[c];
+  const uint32_t cid = ch.cid[c];
+  const int64_t b = ch.base[c];
+  int Z = 0, len = 0;
+  uint4 x = make_uint4(0, 0, 0, 0);
+  for (int i = 1; i <= nk; ++i) {
+    ++len;
+    if (i == nk) { ch.N[b + Z] = len; ++Z; break; }
+    const int wi = i - 1;
+    if ((wi & 3) == 0) x = philox4x32_10(make_uint4((uint32_t)(wi >> 2), 0u, cid, kPurposeInitCut), E.key0, E.key1);
+    const uint32_t word = (wi & 3) == 0 ? x.x : (wi & 3) == 1 ? x.y : (wi & 3) == 2 ? x.z : x.w;
+    if (word < P.cut_threshold) { ch.N[b + Z] = len; ++Z; len = 0; }
+  }
+  for (int k = 0; k < Z; ++k) {
+    float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+    if (k > 0 && k < Z - 1) {
+      const uint4 g = philox4x32_10(make_uint4((uint32_t)k, 0u, cid, kPurposeInitQ), E.key0, E.key1);
+      float g0, g1, g2, g3;
+      boxmuller(g.x, g.y, g0, g1);
+      boxmuller(g.z, g.w, g2, g3);
+      const float sig = sqrtf((float)ch.N[b + k] / 3.0f);
+      qx = g0 * sig; qy = g1 * sig; qz = g2 * sig;
+    }
+    ch.Qx[b + k] = qx; ch.Qy[b + k] = qy; ch.Qz[b + k] = qz;
+    ch.w[b + k] = 0.0f;
+  }
+  if (E.cd) {
+    for (int k = 0; k < Z - 1; ++k) {
+      if ((k & 3) == 0) x = philox4x32_10(make_uint4((uint32_t)(k >> 2), 0u, cid, kPurposeInitTau), E.key0, E.key1);
+      const uint32_t word = (k & 3) == 0 ? x.x : (k & 3) == 1 ? x.y : (k & 3) == 2 ? x.z : x.w;
+      ch.w[b + k] = rate_ft(E, u01(word));
+    }
+  }
+  ch.Z[c] = Z; ch.stall[c] = 0; ch.t[c] = 0.0; ch.ev[c] = 0; ch.dt_prev[c] = 0.0f;
+  ch.next_sample[c] = (P.sample_mode == kSampleFlow) ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ utility kernels
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void iota_kernel(int* p, int start, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = start + i;
+}
+
+// flow stress: per (sample, ensemble) FP64 sums of the 8 components over healthy
+// chains (stall == 0, finite, recorded); out[...*9 + 8] = chain count.
+// (common/ensemble_call_block.cu:452-505 averages in FP32 on the host.)
+__global__ void __launch_bounds__(256) flow_reduce_kernel(const float* stage, const int* stall, int chain0,
+                                                            int nchains, int nrows, double* out) {
+  const int row = blockIdx.x;
+  if (row >= nrows) return;
+  __shared__ double red[9][256];
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  for (int c = threadIdx.x; c < nchains; c += blockDim.x) {
+    const float* o = stage + ((int64_t)row * nchains + c) * 8;
+    bool ok = stall[chain0 + c] == 0 && o[7] > 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ok = ok && (o[q] == o[q]);
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] += (double)o[q];
+      acc[8] += 1.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) out[(int64_t)row * 9 + q] = red[q][0];
+  }
+}
+
+// ------------------------------------------------------------------ multi-tau correlator
+// Thread per chain (lanes = consecutive chains -> every access below is coalesced).
+// Per-chain FP64 state: D[level][slot][3], A[level][3], C[level][lag]
+// (C accumulates the sum of the three component products).  The level
+// counters are identical for all chains (sampling is synchronous) and are
+// passed in; the host advances its copy by the same rule after each launch.
+// Insert rule, block averages and lag products: common/eq_ensemble_kernel.cu:133-227.
+struct CorrArgs {
+  const float* stage;   // [k][chain][3]
+  double* D;            // [((lvl*p + slot)*3 + comp)*nch + chain]
+  double* A;            // [(lvl*3 + comp)*nch + chain]
+  double* C;            // [(lvl*p + lag)*nch + chain]
+  const int* stall;     // global stall flags
+  int chain0, nch, p, m, s, nrows;
+  const int* cnt;       // [3*s]: ins, macc, nins (clamped to p) at window start
+};
+
+__global__ void __launch_bounds__(256) corr_update_kernel(CorrArgs a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nch) return;
+  int ins[40], macc[40], nins[40];
+  for (int k = 0; k < a.s; ++k) { ins[k] = a.cnt[k]; macc[k] = a.cnt[a.s + k]; nins[k] = a.cnt[2 * a.s + k]; }
+  const int64_t nch = a.nch;
+  const int p = a.p, m = a.m;
+  for (int row = 0; row < a.nrows; ++row) {
+    const float* o = a.stage + ((int64_t)row * nch + c) * 3;
+    double vx = o[0], vy = o[1], vz = o[2];
+    int lvl = 0;
+    for (;;) {
+      if (lvl == a.s) break;
+      const int in = ins[lvl];
+      double* D = a.D + (int64_t)(lvl * p) * 3 * nch + c;
+      D[(int64_t)(in * 3 + 0) * nch] = vx;
+      D[(int64_t)(in * 3 + 1) * nch] = vy;
+      D[(int64_t)(in * 3 + 2) * nch] = vz;
+      double* A = a.A + (int64_t)(lvl * 3) * nch + c;
+      A[0] += vx; A[nch] += vy; A[2 * nch] += vz;
+      macc[lvl]++;
+      const bool push = macc[lvl] == m;
+      double ux = 0.0, uy = 0.0, uz = 0.0;
+      if (push) {
+        ux = A[0] / m; uy = A[nch] / m; uz = A[2 * nch] / m;
+        A[0] = 0.0; A[nch] = 0.0; A[2 * nch] = 0.0;
+        macc[lvl] = 0;
+      }
+      const int jmin = (lvl == 0) ? 0 : p / m;
+      double* Cc = a.C + (int64_t)(lvl * p) * nch + c;
+      for (int j = jmin; j < p; ++j) {
+        if (j > nins[lvl]) break;
+        int idx = in - j;
+        if (idx < 0) idx += p;
+        const double dx = D[(int64_t)(idx * 3 + 0) * nch];
+        const double dy = D[(int64_t)(idx * 3 + 1) * nch];
+        const double dz = D[(int64_t)(idx * 3 + 2) * nch];
+        Cc[(int64_t)j * nch] += vx * dx + vy * dy + vz * dz;
+      }
+      ins[lvl] = (in + 1 == p) ? 0 : in + 1;
+      nins[lvl] = nins[lvl] < p ? nins[lvl] + 1 : p;
+      if (!push) break;
+      ++lvl;
+      vx = ux; vy = uy; vz = uz;
+    }
+  }
+}
+
+// G(level, lag) = sum over healthy chains of C / (3 * n_pairs * n_healthy)
+__global__ void __launch_bounds__(256) corr_reduce_kernel(const double* C, const int* stall, int chain0, int nch,
+                                                            int npts, double* out_sum, int* out_healthy) {
+  const int pt = blockIdx.x;
+  if (pt >= npts) return;
+  __shared__ double red[256];
+  __shared__ int cnt[256];
+  double acc = 0.0;
+  int h = 0;
+  for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+    if (stall[chain0 + c] == 0) { acc += C[(int64_t)pt * nch + c]; ++h; }
+  }
+  red[threadIdx.x] = acc;
+  cnt[threadIdx.x] = h;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { red[threadIdx.x] += red[threadIdx.x + s]; cnt[threadIdx.x] += cnt[threadIdx.x + s]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out_sum[pt] = red[0]; if (pt == 0) *out_healthy = cnt[0]; }
+}
+
+// ------------------------------------------------------------------ scan probe (tests)
+__global__ void scan_probe_kernel(const float* in, float* out, int n) {
+  const int s = blockIdx.x;
+  if (s * 64 >= n) return;
+  const int i = s * 64 + (int)threadIdx.x;
+  const float v = i < n ? in[i] : 0.0f;
+  const float r = wave_scan(v);
+  if (i < n) out[i] = r;
+}
+
+__global__ void math_probe_kernel(const float* in, float* out, int n, int which) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = in[i];
+  float r = 0.0f;
+  if (which == 0) r = dexpf(x);
+  else if (which == 1) r = dlogf(x);
+  else if (which == 2) { float s, c; dsincos2pif(x, s, c); r = s; }
+  else if (which == 3) { float s, c; dsincos2pif(x, s, c); r = c; }
+  out[i] = r;
+}
+
+__global__ void philox_probe_kernel(const uint32_t* ctr_key, uint32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t* q = ctr_key + 6 * i;
+  const uint4 r = philox4x32_10(make_uint4(q[0], q[1], q[2], q[3]), q[4], q[5]);
+  out[4 * i + 0] = r.x; out[4 * i + 1] = r.y; out[4 * i + 2] = r.z; out[4 * i + 3] = r.w;
+}
+
+}  // namespace dsm
+
+// ------------------------------------------------------------------ launchers
+#include "dsm_launch.h"
+
+namespace dsm {
+
+hipError_t launch_kmc(const KmcArgs& a, int S, bool plain, int grid, hipStream_t st) {
+  switch (S) {
+    case 1: return launch_kmc_slots_1(a, plain, grid, st);
+    case 2: return launch_kmc_slots_2(a, plain, grid, st);
+    case 4: return launch_kmc_slots_4(a, plain, grid, st);
+    case 8: return launch_kmc_slots_8(a, plain, grid, st);
+    case 16: return launch_kmc_slots_16(a, plain, grid, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int kmc_max_blocks_per_cu(int S, bool plain) {
+  switch (S) {
+    case 1: return kmc_occupancy_slots_1(plain);
+    case 2: return kmc_occupancy_slots_2(plain);
+    case 4: return kmc_occupancy_slots_4(plain);
+    case 8: return kmc_occupancy_slots_8(plain);
+    case 16: return kmc_occupancy_slots_16(plain);
+    default: return 0;
+  }
+}
+
+hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int* chains, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(init_eq_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ch, ens, chains, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (uint32_t*)p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_iota(int* p, int start, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, start, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_flow_reduce(const float* stage, const int* stall, int chain0, int nch, int nrows, double* out,
+                              hipStream_t st) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(flow_reduce_kernel, dim3(nrows), dim3(256), 0, st, stage, stall, chain0, nch, nrows, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_corr_update(const float* stage, double* D, double* A, double* C, const int* stall, int chain0,
+                              int nch, int p, int m, int s, int nrows, const int* cnt, hipStream_t st) {
+  if (nrows <= 0 || nch <= 0) return hipSuccess;
+  CorrArgs a{stage, D, A, C, stall, chain0, nch, p, m, s, nrows, cnt};
+  hipLaunchKernelGGL(corr_update_kernel, dim3((nch + 255) / 256), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_corr_reduce(const double* C, const int* stall, int chain0, int nch, int npts, double* out_sum,
+                              int* out_healthy, hipStream_t st) {
+  if (npts <= 0) return hipSuccess;
+  hipLaunchKernelGGL(corr_reduce_kernel, dim3(npts), dim3(256), 0, st, C, stall, chain0, nch, npts, out_sum,
+                     out_healthy);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_probe(const float* in, float* out, int n, hipStream_t st) {
+  hipLaunchKernelGGL(scan_probe_kernel, dim3((n + 63) / 64), dim3(64), 0, st, in, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_math_probe(const float* in, float* out, int n, int which, hipStream_t st) {
+  hipLaunchKernelGGL(math_probe_kernel, dim3((n + 255) / 256), dim3(256), 0, st, in, out, n, which);
+  return hipGetLastError();
+}
+hipError_t launch_philox_probe(const uint32_t* ck, uint32_t* out, int n, hipStream_t st) {
+  hipLaunchKernelGGL(philox_probe_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ck, out, n);
+  return hipGetLastError();
+}
+
+}  // namespace dsm

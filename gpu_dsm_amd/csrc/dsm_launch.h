@@ -1,0 +1,36 @@
+// dsm_launch.h -- host-callable launchers for the kernels in dsm_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dsm_types.h"
+
+namespace dsm {
+
+// KMC: S = register slots per lane (1,2,4,8,16,32); plain = one jump per chain
+// per launch, grid = number of work items; batched = persistent, grid = blocks.
+hipError_t launch_kmc(const KmcArgs& a, int S, bool plain, int grid, hipStream_t st);
+#define DSM_DECLARE_SLOTS(S)                                                                   \
+  hipError_t launch_kmc_slots_##S(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
+  int kmc_occupancy_slots_##S(bool plain);
+DSM_DECLARE_SLOTS(1)
+DSM_DECLARE_SLOTS(2)
+DSM_DECLARE_SLOTS(4)
+DSM_DECLARE_SLOTS(8)
+DSM_DECLARE_SLOTS(16)
+#undef DSM_DECLARE_SLOTS
+int kmc_max_blocks_per_cu(int S, bool plain);
+hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int* chains, int n, hipStream_t st);
+hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st);
+hipError_t launch_iota(int* p, int start, int n, hipStream_t st);
+hipError_t launch_flow_reduce(const float* stage, const int* stall, int chain0, int nch, int nrows, double* out,
+                              hipStream_t st);
+hipError_t launch_corr_update(const float* stage, double* D, double* A, double* C, const int* stall, int chain0,
+                              int nch, int p, int m, int s, int nrows, const int* cnt, hipStream_t st);
+hipError_t launch_corr_reduce(const double* C, const int* stall, int chain0, int nch, int npts, double* out_sum,
+                              int* out_healthy, hipStream_t st);
+hipError_t launch_scan_probe(const float* in, float* out, int n, hipStream_t st);
+hipError_t launch_math_probe(const float* in, float* out, int n, int which, hipStream_t st);
+hipError_t launch_philox_probe(const uint32_t* ck, uint32_t* out, int n, hipStream_t st);
+
+}  // namespace dsm

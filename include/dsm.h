@@ -1,0 +1,167 @@
+/*
+ * dsm.h -- C ABI of gpu_dsm_amd, the MI355X (gfx950) Discrete Slip-Link Model
+ * ensemble KMC engine.  Plain C types only; no HIP or torch types cross it.
+ *
+ * Each entry point names the reference (ktaletsk/gpu_dsm) interface it replaces.
+ * The reference exposes one non-reentrant, global-state ensemble per process
+ * (common/ensemble.h:54-77) driven by main_cuda (common/main_cuda.cu:34); this
+ * ABI is handle based so several ensembles (jobs) can share one GPU.
+ *
+ * Errors: every call returns DSM_OK (0) or a negative DSM_ERR_* code and never
+ * calls exit(); dsm_last_error() describes the last failure of the calling thread.
+ * A handle may be used by one thread at a time; separate handles are independent.
+ */
+#ifndef GPU_DSM_AMD_DSM_H
+#define GPU_DSM_AMD_DSM_H
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSM_OK 0
+#define DSM_ABORTED (-1)      /* run_flag cleared; reference returns -1 (common/ensemble_call_block.cu:214,353) */
+#define DSM_ERR_ARG (-2)
+#define DSM_ERR_DEVICE (-3)   /* reference exits with code 1 (common/cudautil.cu:33-36) */
+#define DSM_ERR_MEMORY (-4)
+#define DSM_ERR_IO (-5)
+#define DSM_ERR_MISMATCH (-6) /* -l file holds a different N_cha; reference exit(2) (common/ensemble.cu:493-496) */
+#define DSM_ERR_STATE (-7)
+#define DSM_ERR_NOKERNEL (-8) /* HIP code object missing or not loadable on this device */
+
+#define DSM_SAMPLE_NONE 0
+#define DSM_SAMPLE_GT 1       /* off-diagonal stress samples for G(t) (common/eq_ensemble_kernel.cu:281-296) */
+#define DSM_SAMPLE_FLOW 2     /* full stress every delta (common/ensemble_kernel.cu:504-538) */
+
+#define DSM_MODE_BATCHED 0    /* persistent, register-resident, Z-binned work queue */
+#define DSM_MODE_PLAIN 1      /* one jump per chain per launch, state in HBM */
+
+typedef struct dsm_engine dsm_engine;
+
+/* One ensemble (the contents of input.dat, common/main_cuda.cu:53-68, plus the
+   new knobs kept out of input.dat). */
+typedef struct {
+  int32_t n_chains;
+  int32_t nk;                   /* Kuhn steps per chain (monodisperse) */
+  const int32_t* nk_per_chain;  /* optional per-chain Nk (polydisperse); NULL = nk */
+  float beta;
+  int32_t cd;                   /* constraint dynamics 0/1 */
+  float kappa[9];               /* velocity gradient, row-major xx xy xz yx yy yz zx zy zz */
+  int32_t exp_wait;             /* 0: dt = 1/W as the reference; 1: exponential waiting times */
+  uint32_t seed;                /* Philox key word 0 (the job_ID for the CLI) */
+  uint32_t salt;                /* Philox key word 1 (distinguishes jobs sharing a seed) */
+  int32_t sample_mode;          /* DSM_SAMPLE_* */
+  double delta;                 /* sample (G) or output (flow) interval in tau_K */
+  int32_t corr_p;               /* multi-tau points per level (reference 64) */
+  int32_t corr_m;               /* multi-tau block factor (reference 8) */
+  int32_t corr_levels;          /* levels s; 0 = derive from the run length */
+  int32_t window;               /* samples per staging window; 0 = default */
+} dsm_ensemble_desc;
+
+typedef struct {
+  uint64_t events;      /* KMC jumps performed */
+  uint64_t z_sum;       /* sum over jumps of the chain's Z (algorithmic byte model) */
+  double kmc_ms;        /* device time of the KMC kernel launches (HIP events) */
+  double first_kmc_ms;  /* device time of the first-bin (slot count 1 or the smallest) launches only */
+  double total_ms;      /* host wall time of the call */
+  int64_t kmc_launches;
+  int64_t first_launches;
+} dsm_run_stats;
+
+/* Device selection and engine set-up; replaces checkCUDA (common/cudautil.cu:27-53)
+   and the device half of gpu_init (common/ensemble.cu:118-208). */
+int dsm_create(int32_t device_id, dsm_engine** out);
+/* Frees the engine's device memory and streams; replaces gpu_clean
+   (common/ensemble.cu:506-539) without resetting the device. */
+int dsm_destroy(dsm_engine* e);
+const char* dsm_last_error(void);
+/* free/total HBM bytes of a device (serving admission). */
+int dsm_mem_info(int32_t device_id, size_t* free_bytes, size_t* total_bytes);
+
+/* Allocates an ensemble's chain state and tables; replaces gpu_init/init_call_block
+   (common/ensemble.cu:118-208, common/ensemble_call_block.cu:55-104) and the
+   p_CD set-up (common/pcd_tau.h:32-63, common/gpu_random.cu:59-104). */
+int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id);
+/* Releases an ensemble's slots for reuse by later ensembles. */
+int dsm_remove_ensemble(dsm_engine* e, int32_t ens);
+/* Equilibrium start; replaces host_chains_init/chain_init (common/ensemble.cu:106-115,
+   common/chain.cu:131-215).  Runs on the GPU, keyed on Philox(seed, salt). */
+int dsm_init_equilibrium(dsm_engine* e, int32_t ens);
+
+/* Chain state in caller-owned host buffers, strand stride `cap` (>= max Nk):
+   Z[n], N[n*cap], Q[n*cap*3], w[n*cap], t[n], ev[n], stall[n], dt_prev[n],
+   next_sample[n]; any pointer may be NULL.  Replaces get_chains_from_device /
+   get_chain_to_device_call_block (common/ensemble.cu:231-236,
+   common/ensemble_call_block.cu:402-450). */
+int dsm_state_cap(dsm_engine* e, int32_t ens, int32_t* cap);
+int dsm_get_state(dsm_engine* e, int32_t ens, int32_t cap, int32_t* Z, int32_t* N, float* Q, float* w,
+                  double* t, uint64_t* ev, int32_t* stall, float* dt_prev, int64_t* next_sample);
+int dsm_set_state(dsm_engine* e, int32_t ens, int32_t cap, const int32_t* Z, const int32_t* N, const float* Q,
+                  const float* w, const double* t, const uint64_t* ev, const int32_t* stall,
+                  const float* dt_prev, const int64_t* next_sample);
+
+/* Advances every chain of the ensemble until its clock reaches t_end, recording
+   samples and updating the correlator / flow stress on the way.  run_flag (may be
+   NULL) is polled between launches; progress (may be NULL) receives 0..100.
+   Replaces gpu_time_step/time_step_call_block (common/ensemble.cu:223-229,
+   common/ensemble_call_block.cu:106-222) and the G(t) loop of gpu_Gt_PCS /
+   EQ_time_step_call_block (common/ensemble.cu:289-339,
+   common/ensemble_call_block.cu:224-379). */
+int dsm_advance(dsm_engine* e, int32_t ens, double t_end, int32_t mode, volatile const int32_t* run_flag,
+                volatile int32_t* progress, dsm_run_stats* stats);
+/* Several ensembles (different Nk, beta, kappa, targets) in one batched launch
+   sequence: the persistent kernels take chains of all of them from one queue. */
+int dsm_advance_many(dsm_engine* e, const int32_t* ens_ids, const double* t_end, int32_t count, int32_t mode,
+                     volatile const int32_t* run_flag, dsm_run_stats* stats);
+/* Exactly n_events jumps per chain (no time target); for parity tests and benches. */
+int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mode, dsm_run_stats* stats);
+
+/* Flow stress rows recorded so far: out[row*9 + q] = sum over healthy chains of
+   xx yy zz xy yz zx Lpp Z (q < 8) and the healthy-chain count (q = 8), row r is
+   time (r+1)*delta.  Replaces calc_stress/calc_stress_call_block
+   (common/ensemble.cu:210-221, common/ensemble_call_block.cu:452-505). */
+int dsm_flow_stress(dsm_engine* e, int32_t ens, int64_t first_row, int64_t max_rows, double* out,
+                    int64_t* n_rows);
+/* G(t): lags in samples and G values (np points, reference order); replaces
+   correlator::calc and the accumulation in gpu_Gt_PCS (common/correlator.cu:146-187,
+   common/ensemble.cu:309-321). */
+int dsm_correlator(dsm_engine* e, int32_t ens, int32_t max_pts, double* lag, double* g, int32_t* npts);
+/* The samples of the last staging window: out[(k-k0)*n*3 + chain*3 + comp]. */
+int dsm_last_samples(dsm_engine* e, int32_t ens, float* out, int64_t max_floats, int64_t* k0, int64_t* nrows);
+
+/* Inspection of the FP32 tables / p_CD constants an ensemble runs with
+   (which: 0 SP 1 SM 2 PP 3 PM 4 EC 5 ED 6 ED2 7 T3 8 CC 9 FR). */
+int dsm_tables(dsm_engine* e, int32_t ens, int32_t which, float* out, int32_t* n);
+int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* out24);
+
+/* Building-block probes (run one kernel on the device): DPP wave scan over
+   64-position slots, exp/log/sin/cos (which 0..3), Philox (6 words in, 4 out). */
+int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n);
+int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which);
+int dsm_probe_philox(int32_t device_id, const uint32_t* ctr_key, uint32_t* out, int32_t n);
+
+/* Options of the drop-in driver that input.dat cannot carry. */
+typedef struct {
+  int32_t mode;         /* DSM_MODE_* */
+  int32_t corr_p;       /* default 64 */
+  int32_t corr_m;       /* default 8 */
+  int32_t window;       /* 0 = default */
+  int32_t exp_wait;     /* default 0 */
+  int32_t quiet;        /* suppress per-line stdout echo */
+  const char* workdir;  /* directory holding input.dat and receiving outputs; NULL = cwd */
+} dsm_main_options;
+
+/* Drop-in for main_cuda (common/main_cuda.cu:34): reads input.dat, runs the
+   flow / G(t) calculation, writes tau/G/MSD[_ID].dat, distr_*.dat and the -s file.
+   Returns 0, -1 when run_flag is cleared, or a DSM_ERR_* code. */
+int dsm_main(volatile const bool* run_flag, int job_ID, const char* savefile, const char* loadfile,
+             int device_ID, bool distr, volatile int* progress_bar);
+int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char* savefile, const char* loadfile,
+                int device_ID, bool distr, volatile int* progress_bar, const dsm_main_options* opt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,193 @@
+"""HIP path vs the CPU oracle (mirror-FP32 mode), through the C ABI.
+
+Integer chain state (Z, every N_i, event counts) and the FP32 strand vectors,
+CD rates and FP64 clocks must be bit-identical after a fixed number of jumps.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_same_state(g, o, what=""):
+    Z = g["Z"]
+    assert np.array_equal(Z, o["Z"]), "%s: Z differs in %d chains (first %s)" % (
+        what, int((Z != o["Z"]).sum()), np.nonzero(Z != o["Z"])[0][:5])
+    cap = min(g["N"].shape[1], o["N"].shape[1])
+    k = np.arange(cap)[None, :]
+    on = k < Z[:, None]
+    link = k < (Z[:, None] - 1)
+    gN, oN = np.where(on, g["N"][:, :cap], 0), np.where(on, o["N"][:, :cap], 0)
+    bad = np.nonzero((gN != oN).any(axis=1))[0]
+    assert bad.size == 0, "%s: N differs in chains %s" % (what, bad[:5])
+    gQ = np.where(on[..., None], _bits(g["Q"][:, :cap]), 0)
+    oQ = np.where(on[..., None], _bits(o["Q"][:, :cap]), 0)
+    bad = np.nonzero((gQ != oQ).any(axis=(1, 2)))[0]
+    assert bad.size == 0, "%s: Q bits differ in chains %s" % (what, bad[:5])
+    gw = np.where(link, _bits(g["w"][:, :cap]), 0)
+    ow = np.where(link, _bits(o["w"][:, :cap]), 0)
+    bad = np.nonzero((gw != ow).any(axis=1))[0]
+    assert bad.size == 0, "%s: CD rates differ in chains %s" % (what, bad[:5])
+    assert np.array_equal(g["t"], o["t"]), "%s: FP64 clocks differ (max |d| %g)" % (
+        what, float(np.abs(g["t"] - o["t"]).max()))
+    assert np.array_equal(g["ev"], o["ev"]), "%s: event counters differ" % what
+
+
+# ------------------------------------------------------------------ building blocks
+def test_philox_matches_rocrand_vectors():
+    import gpu_dsm_amd as g
+    vec = json.load(open(os.path.join(GOLDEN, "philox_rocrand.json")))
+    ck = np.array([v["ctr"] + v["key"] for v in vec], np.uint32)
+    out = g.probe_philox(ck)
+    assert np.array_equal(out, np.array([v["out"] for v in vec], np.uint32))
+
+
+def test_wave_scan_tree_matches_oracle(oracle):
+    import gpu_dsm_amd as g
+    rng = np.random.default_rng(5)
+    for vals in (rng.random(64 * 7).astype(np.float32),
+                 (rng.standard_normal(64 * 5) * 10).astype(np.float32),
+                 rng.exponential(size=64 * 3).astype(np.float32) * np.float32(1e3)):
+        gpu = g.probe_scan(vals)
+        ref = np.concatenate([oracle.scan_tree(vals[i:i + 64]) for i in range(0, len(vals), 64)])
+        assert np.array_equal(_bits(gpu), _bits(ref))
+
+
+@pytest.mark.parametrize("which", ["exp", "log", "sin2pi", "cos2pi"])
+def test_math_bit_exact(oracle, which):
+    import gpu_dsm_amd as g
+    rng = np.random.default_rng(11)
+    if which == "exp":
+        x = np.concatenate([np.linspace(-90, 90, 200001), rng.standard_normal(100000) * 3,
+                            [0.0, -0.0, 88.0, -87.0, 1e-30, -1e-30]]).astype(np.float32)
+    elif which == "log":
+        x = np.concatenate([((np.arange(0, 1 << 24, 97) + 0.5) * 2.0 ** -24),
+                            np.exp(rng.uniform(-80, 80, 100000)), [1.0, 2.0, 0.5, 3.4e38]]).astype(np.float32)
+    else:
+        x = ((np.arange(0, 1 << 24, 53) + 0.5) * 2.0 ** -24).astype(np.float32)
+    assert np.array_equal(_bits(g.probe_math(x, which)), _bits(oracle.vec_math(which, x)))
+
+
+# ------------------------------------------------------------------ equilibrium init
+INIT_CASES = [
+    dict(n=500, nk=25, beta=1.0, cd=0),
+    dict(n=300, nk=100, beta=1.0, cd=1),
+    dict(n=200, nk=60, beta=0.5, cd=1),
+    dict(n=64, nk=300, beta=2.0, cd=1),
+]
+
+
+@pytest.mark.parametrize("case", INIT_CASES, ids=lambda c: "nk%d_b%g_cd%d" % (c["nk"], c["beta"], c["cd"]))
+def test_init_parity(engine, oracle, case):
+    P = oracle.Params(case["beta"], case["nk"], case["cd"])
+    ens = engine.add_ensemble(case["n"], case["nk"], case["beta"], case["cd"], seed=7, salt=3)
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=7, n=case["n"], salt=3)
+    assert_same_state(engine.state(ens), O.state(), "init")
+    engine.remove_ensemble(ens)
+
+
+# ------------------------------------------------------------------ KMC jumps
+KMC_CASES = [
+    dict(n=300, nk=25, beta=1.0, cd=0, events=400),
+    dict(n=200, nk=100, beta=1.0, cd=1, events=300),
+    dict(n=128, nk=50, beta=0.5, cd=1, events=300),
+    dict(n=96, nk=160, beta=1.0, cd=1, events=250),
+    dict(n=48, nk=300, beta=1.0, cd=1, events=200),
+    dict(n=128, nk=40, beta=2.0, cd=1, events=300, exp_wait=True),
+    dict(n=128, nk=30, beta=1.0, cd=1, events=300, kappa=[0, 0.05, 0, 0, 0, 0, 0, 0, 0]),
+    dict(n=64, nk=20, beta=1.0, cd=1, events=300, kappa=[0.02, 0, 0, 0, -0.01, 0, 0, 0, -0.01]),
+]
+
+
+def _case_id(c):
+    s = "nk%d_b%g_cd%d" % (c["nk"], c["beta"], c["cd"])
+    if c.get("exp_wait"):
+        s += "_expwait"
+    if c.get("kappa"):
+        s += "_flow"
+    return s
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+@pytest.mark.parametrize("case", KMC_CASES, ids=_case_id)
+def test_kmc_parity(engine, oracle, case, mode):
+    kappa = case.get("kappa")
+    P = oracle.Params(case["beta"], case["nk"], case["cd"], kappa=kappa, exp_wait=case.get("exp_wait", False))
+    ens = engine.add_ensemble(case["n"], case["nk"], case["beta"], case["cd"], kappa=kappa, seed=11, salt=1,
+                              exp_wait=case.get("exp_wait", False))
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=11, n=case["n"], salt=1)
+    half = case["events"] // 2
+    for k, n_ev in enumerate((half, case["events"] - half)):
+        st = engine.advance_events(ens, n_ev, mode=mode)
+        ev, _ = O.advance(max_events=n_ev)
+        assert st["events"] == ev, "event totals differ (%d vs %d)" % (st["events"], ev)
+        assert_same_state(engine.state(ens), O.state(), "%s after chunk %d" % (mode, k))
+    engine.remove_ensemble(ens)
+
+
+def test_batched_time_target_parity(engine, oracle):
+    """Run to a time target (persistent form with Z-binning) and compare."""
+    P = oracle.Params(1.0, 100, 1)
+    ens = engine.add_ensemble(256, 100, 1.0, 1, seed=3)
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=3, n=256)
+    for t in (5.0, 12.5, 30.0):
+        engine.advance(ens, t)
+        O.advance(t_end=t)
+        assert_same_state(engine.state(ens), O.state(), "t=%g" % t)
+    engine.remove_ensemble(ens)
+
+
+# ------------------------------------------------------------------ samples, G(t), flow stress
+def test_gt_samples_and_correlator(engine, oracle):
+    n, nk, T = 200, 25, 300.0
+    p, m = 16, 2
+    s = oracle.corr_levels(T, 1.0, p, m)
+    P = oracle.Params(1.0, nk, 0)
+    ens = engine.add_ensemble(n, nk, 1.0, 0, seed=1, sample="gt", delta=1.0, corr_p=p, corr_m=m,
+                              corr_levels=s, window=64)
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=1, n=n)
+    _, samples = O.advance(t_end=T, delta=1.0, k0=0, k_end=int(T), samples=True)
+    engine.advance(ens, T)
+    assert_same_state(engine.state(ens), O.state(), "G(t) run")
+    k0, last = engine.last_samples(ens)
+    assert np.array_equal(_bits(last), _bits(samples[k0:k0 + last.shape[0]]))
+    corr = oracle.Correlator(p, m, s, n)
+    for row in samples:
+        corr.add_row(row)
+    lag_o, g_o = corr.result()
+    lag, g = engine.correlator(ens)
+    assert np.array_equal(lag, lag_o)
+    np.testing.assert_allclose(g, g_o, rtol=1e-10, atol=1e-12)
+    engine.remove_ensemble(ens)
+
+
+def test_flow_stress_rows(engine, oracle):
+    n, nk, T, dlt = 128, 30, 20.0, 2.0
+    kappa = [0, 0.02, 0, 0, 0, 0, 0, 0, 0]
+    P = oracle.Params(1.0, nk, 1, kappa=kappa)
+    ens = engine.add_ensemble(n, nk, 1.0, 1, kappa=kappa, seed=5, sample="flow", delta=dlt, window=4)
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=5, n=n)
+    k_end = int(T / dlt) + 1
+    _, samples = O.advance(t_end=T, delta=dlt, k0=1, k_end=k_end, samples=True, flow_samples=True)
+    engine.advance(ens, T)
+    assert_same_state(engine.state(ens), O.state(), "flow run")
+    rows = engine.flow_stress(ens)
+    assert rows.shape[0] == k_end - 1
+    ref = samples.astype(np.float64).sum(axis=1)
+    np.testing.assert_allclose(rows[:, :8], ref, rtol=1e-12, atol=1e-9)
+    assert np.all(rows[:, 8] == n)
+    engine.remove_ensemble(ens)
