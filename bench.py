@@ -1,0 +1,165 @@
+"""Benchmark: KMC jumps/s of the DSM ensemble path on one MI355X.
+
+Workload (the single-GPU config the metric is quoted on): monodisperse equilibrium
+melt of 131,072 chains, Nk=100, beta=1, constraint dynamics on, G(t) sampling every
+tau_K into the multi-tau correlator (16 points/level, block factor 2).  One "step"
+advances every chain by `--window` Kuhn times (default 16 samples) through the batched
+persistent kernels plus the correlator update.  The plain form (one jump per chain per
+launch) is measured on the same state beside it.
+
+Printed: one JSON line.  value = whole-job jumps/s of the batched form with the chain
+state already resident in HBM.  roofline uses the algorithmic bytes per jump
+B(Z) = 20*Z + 32 (strand records Q,N,w plus the chain header) over the device time of
+the dominant kernel (the single-slot batched KMC kernel), measured with HIP events on
+the engine's stream.  cpu_baseline is the CPU oracle (same algorithm, same Philox
+streams) timed on this host on a bounded chain subset.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+WORKLOAD = dict(n_chains=131072, nk=100, beta=1.0, cd=1, corr_p=16, corr_m=2, delta=1.0)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--window", type=int, default=16, help="Kuhn times (samples) per step")
+    ap.add_argument("--chains", type=int, default=WORKLOAD["n_chains"])
+    ap.add_argument("--plain-steps", type=int, default=64, help="plain-form launches to time")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "kmc_batched_s1_c2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(args):
+    from oracle import oracle as O
+    w = WORKLOAD
+    P = O.Params(w["beta"], w["nk"], w["cd"])
+    threads = min(os.cpu_count() or 1, 16)
+    # calibrate the subset size to roughly args.cpu_seconds of work
+    n, t_end = 256, 20.0
+    ev, sec = O.bench(P, args.seed, n, t_end, threads)
+    rate = ev / max(sec, 1e-9)
+    n_target = int(min(16384, max(256, n * args.cpu_seconds / max(sec, 1e-3) / 1.2)))
+    ev, sec = O.bench(P, args.seed, n_target, t_end, threads)
+    rate = ev / sec
+    return {"value": rate, "unit": "jumps/s", "cores": threads, "kind": "port",
+            "sample": "first %d chains of the same ensemble (same Philox streams) advanced to t=%g tau_K "
+                      "with the mirror-FP32 oracle on %d host threads: %d jumps in %.1f s"
+                      % (n_target, t_end, threads, ev, sec)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
+              "(the path shards by chain ranges across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import gpu_dsm_amd as g
+
+    w = WORKLOAD
+    eng = g.Engine(0)
+    T_total = float(args.window * (args.warmup + args.steps + 1))
+    levels = 0
+    import math
+    levels = int(math.ceil(math.log(T_total / w["delta"] / w["corr_p"]) / math.log(w["corr_m"]))) + 1
+    ens = eng.add_ensemble(args.chains, w["nk"], w["beta"], w["cd"], seed=args.seed, sample="gt", delta=w["delta"],
+                           corr_p=w["corr_p"], corr_m=w["corr_m"], corr_levels=max(levels, 2),
+                           window=args.window)
+    eng.init_equilibrium(ens)
+    t = 0.0
+    for _ in range(args.warmup):
+        t += args.window
+        eng.advance(ens, t)
+    ev = zs = 0
+    kmc_ms = first_ms = 0.0
+    first_launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t += args.window
+        st = eng.advance(ens, t)
+        ev += st["events"]
+        zs += st["z_sum"]
+        kmc_ms += st["kmc_ms"]
+        first_ms += st["first_kmc_ms"]
+        first_launches += st["first_launches"]
+    wall = time.perf_counter() - t0
+    value = ev / wall
+    bytes_alg = 20.0 * zs + 32.0 * ev
+    achieved = bytes_alg / (first_ms * 1e-3) / 1e9 if first_ms > 0 else 0.0
+
+    # plain form on the same state: one jump per chain per launch
+    pst = eng.advance_events(ens, args.plain_steps, mode="plain")
+    plain_rate = pst["events"] / (pst["kmc_ms"] * 1e-3) if pst["kmc_ms"] > 0 else 0.0
+    plain_bytes = 20.0 * pst["z_sum"] + 32.0 * pst["events"]
+    plain_gbs = plain_bytes / (pst["kmc_ms"] * 1e-3) / 1e9 if pst["kmc_ms"] > 0 else 0.0
+    lag, gt = eng.correlator(ens)
+    eng.close()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args)
+        except Exception as exc:  # the baseline is reported, never required
+            cpu = {"value": None, "unit": "jumps/s", "cores": 0, "kind": "port", "sample": "failed: %s" % exc}
+
+    out = {
+        "metric": "KMC jumps/sec & G(t) err vs FP64 oracle, 1 MI355X, batched vs plain",
+        "value": value,
+        "unit": "jumps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (Philox-keyed DSM equilibrium draw)",
+        "config": {"workload": "monodisperse equilibrium melt, %d chains, Nk=%d, beta=%g, CD on, G(t) multi-tau "
+                               "p=%d m=%d, %d tau_K per step" % (args.chains, w["nk"], w["beta"], w["corr_p"],
+                                                                 w["corr_m"], args.window),
+                   "form": "batched persistent (Z-binned work queue, register-resident chains)",
+                   "n_chains": args.chains, "nk": w["nk"], "beta": w["beta"], "cd": w["cd"],
+                   "window_tau_k": args.window},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profile()},
+        "cpu_baseline": cpu,
+        "jumps": ev,
+        "mean_Z_per_jump": zs / max(ev, 1),
+        "kmc_device_ms": kmc_ms,
+        "dominant_kernel_ms": first_ms,
+        "dominant_kernel_launches": first_launches,
+        "dominant_kernel_avg_ms": first_ms / max(first_launches, 1),
+        "algorithmic_bytes_per_jump": bytes_alg / max(ev, 1),
+        "plain": {"jumps_per_s": plain_rate, "achieved_GBs": plain_gbs, "launches": pst["kmc_launches"],
+                  "device_ms": pst["kmc_ms"]},
+        "batched_over_plain": value / plain_rate if plain_rate > 0 else None,
+        "G0": float(gt[0]) if len(gt) else None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -41,6 +41,7 @@ int fail(int code, const char* fmt, ...) {
 constexpr int kSlots[5] = {1, 2, 4, 8, 16};
 constexpr int kNumBins = 5;
 constexpr int kDefaultWindow = 256;
+constexpr int kLdsTableFloats = 2816;  // per-wave LDS budget for rate tables (Nk <= ~200)
 
 // ---------------------------------------------------------------- p_CD (FP32 host, as the reference)
 // common/pcd_tau.h:32-63 and the device constants of common/gpu_random.cu:76-99.
@@ -211,6 +212,7 @@ int grow_chains(dsm_engine* e, int n_new, int64_t s_new) {
       (rc = grow(&c.local, n0, n_new, e->st)) || (rc = grow(&c.base, n0, n_new, e->st)) ||
       (rc = grow(&c.t, n0, n_new, e->st)) || (rc = grow(&c.ev, n0, n_new, e->st)) ||
       (rc = grow(&c.dt_prev, n0, n_new, e->st)) || (rc = grow(&c.next_sample, n0, n_new, e->st)) ||
+      (rc = grow(&c.ev_stop, n0, n_new, e->st)) || (rc = grow(&c.zsum, n0, n_new, e->st)) ||
       (rc = grow(&c.Qx, s0, s_new, e->st)) || (rc = grow(&c.Qy, s0, s_new, e->st)) ||
       (rc = grow(&c.Qz, s0, s_new, e->st)) || (rc = grow(&c.N, s0, s_new, e->st)) ||
       (rc = grow(&c.w, s0, s_new, e->st)))
@@ -321,8 +323,8 @@ struct Timer {
   }
 };
 
-int device_blocks(dsm_engine* e, int S, bool plain) {
-  int nb = kmc_max_blocks_per_cu(S, plain);
+int device_blocks(dsm_engine* e, int S, bool plain, int lds_floats) {
+  int nb = kmc_max_blocks_per_cu(S, plain, lds_floats);
   if (nb <= 0) nb = 1;
   return nb * e->n_cu;
 }
@@ -333,15 +335,25 @@ KmcArgs base_args(dsm_engine* e) {
   a.ens = e->d_ens;
   a.tabs = e->d_tabs;
   a.stage = e->d_stage;
-  a.counters = e->d_counters;
-  a.max_events = INT64_MAX;
+  a.use_ev_stop = 0;
+  a.lds_floats = 0;
   return a;
+}
+
+// sums of the per-chain jump counters and Z sums over the given ensembles
+int chain_totals(dsm_engine* e, const std::vector<int>& ids, unsigned long long out[2]) {
+  HIPCHK(launch_fill_u32(e->d_counters, 0u, 4, e->st));
+  for (int id : ids)
+    HIPCHK(launch_sum_counters(e->ch.ev, e->ch.zsum, e->ens[id].chain0, e->ens[id].n, e->d_counters, e->st));
+  HIPCHK(hipMemcpyAsync(out, e->d_counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  return DSM_OK;
 }
 
 // One KMC pass over every chain of the given ensembles until each reaches its
 // ensemble's P.t_end (batched: persistent Z-binned launches; plain: one jump per
 // launch until no chain moves).  P must already be uploaded.
-int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, int64_t max_events,
+int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_stop,
              volatile const int32_t* run_flag, dsm_run_stats* rs) {
   // work list = all chains of the ensembles, bin 0 gets everything, kernels push overflow up
   int total = 0, bin_max = 0;
@@ -356,7 +368,7 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, int64_t max_e
     off += e->ens[id].n;
   }
   KmcArgs a = base_args(e);
-  a.max_events = max_events;
+  a.use_ev_stop = use_ev_stop ? 1 : 0;
   if (mode == DSM_MODE_PLAIN) {
     // plain: every chain in the slot count of its ensemble's longest possible chain
     const int S = kSlots[bin_max];
@@ -365,11 +377,11 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, int64_t max_e
     a.n_work = cnt;
     a.overflow = e->d_work + (size_t)e->d_work_cap;  // never used at the maximal slot count
     a.n_overflow = cnt + kNumBins;
-    a.max_events = 1;
     const int batch = 16;
     for (;;) {
-      unsigned long long before = 0, after = 0;
-      HIPCHK(hipMemcpyAsync(&before, e->d_counters, sizeof(before), hipMemcpyDeviceToHost, e->st));
+      unsigned long long before[2], after[2];
+      int rc = chain_totals(e, ids, before);
+      if (rc) return rc;
       HIPCHK(hipEventRecord(e->evs[0], e->st));
       for (int b = 0; b < batch; ++b) {
         HIPCHK(launch_kmc(a, S, true, total, e->st));
@@ -377,18 +389,20 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, int64_t max_e
         rs->first_launches++;
       }
       HIPCHK(hipEventRecord(e->evs[1], e->st));
-      HIPCHK(hipMemcpyAsync(&after, e->d_counters, sizeof(after), hipMemcpyDeviceToHost, e->st));
-      HIPCHK(hipStreamSynchronize(e->st));
+      if ((rc = chain_totals(e, ids, after))) return rc;
       float ms = 0.0f;
       HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
       rs->kmc_ms += ms;
       rs->first_kmc_ms += ms;
-      if (after == before) break;
+      if (after[0] == before[0]) break;
       if (run_flag && *run_flag == 0) return DSM_ABORTED;
     }
     return DSM_OK;
   }
-  // batched
+  // batched; tables go to LDS (one copy per wave) when they fit the per-wave budget
+  int tab_max = 0;
+  for (int id : ids) tab_max = std::max(tab_max, 9 * (e->ens[id].nmax + 2) + 4 * e->ens[id].nmax + 4);
+  a.lds_floats = tab_max <= kLdsTableFloats ? tab_max : 0;
   HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumBins, e->st));
   HIPCHK(launch_fill_u32(cnt, (uint32_t)total, 1, e->st));
   for (int b = 0; b <= bin_max; ++b) {
@@ -399,7 +413,7 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, int64_t max_e
     a.overflow = e->d_work + (size_t)nb * e->d_work_cap;
     a.n_overflow = cnt + nb;
     const int S = kSlots[b];
-    const int grid = device_blocks(e, S, false);
+    const int grid = device_blocks(e, S, false, a.lds_floats);
     HIPCHK(hipEventRecord(e->evs[0], e->st));
     HIPCHK(launch_kmc(a, S, false, grid, e->st));
     HIPCHK(hipEventRecord(e->evs[1], e->st));
@@ -414,11 +428,13 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, int64_t max_e
   return DSM_OK;
 }
 
-int finish_stats(dsm_engine* e, dsm_run_stats* rs, const Timer& T) {
+int finish_stats(dsm_engine* e, const std::vector<int>& ids, const unsigned long long before[2],
+                 dsm_run_stats* rs, const Timer& T) {
   unsigned long long c[2];
-  HIPCHK(hipMemcpy(c, e->d_counters, sizeof(c), hipMemcpyDeviceToHost));
-  rs->events = c[0];
-  rs->z_sum = c[1];
+  int rc = chain_totals(e, ids, c);
+  if (rc) return rc;
+  rs->events = c[0] - before[0];
+  rs->z_sum = c[1] - before[1];
   rs->total_ms = T.ms();
   return DSM_OK;
 }
@@ -429,7 +445,8 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
   dsm_run_stats rs{};
   int rc = set_device(e);
   if (rc) return rc;
-  HIPCHK(launch_fill_u32(e->d_counters, 0u, 4, e->st));
+  unsigned long long before[2];
+  if ((rc = chain_totals(e, ids, before))) return rc;
   // per-ensemble sample targets
   std::vector<int64_t> k_target(ids.size(), 0);
   for (size_t q = 0; q < ids.size(); ++q) {
@@ -450,44 +467,38 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
   }
   const double t0_prog = 0.0;
   (void)t0_prog;
+  std::vector<char> reached(ids.size(), 0);
   for (;;) {
-    // next window: each ensemble advances to min(window end, t_end)
-    bool any = false;
+    // next window: each ensemble advances to its window end, the last one to t_end
     std::vector<int> act;
     for (size_t q = 0; q < ids.size(); ++q) {
+      if (reached[q]) continue;
       Ens& E = e->ens[ids[q]];
-      if (E.P.sample_mode == kSampleNone) {
-        if (E.P.t_end != t_end[q] || !any) {}
+      if (E.P.sample_mode == kSampleNone || E.k_cur >= k_target[q]) {
         E.P.t_end = t_end[q];
         E.P.k0 = E.k_cur;
         E.P.k_end = E.k_cur;
         act.push_back(ids[q]);
-        continue;
-      }
-      if (E.k_cur >= k_target[q]) {
-        // samples done; still bring the clock to t_end
-        E.P.k0 = E.k_cur;
-        E.P.k_end = E.k_cur;
-        E.P.t_end = t_end[q];
-        act.push_back(ids[q]);
+        reached[q] = 1;
         continue;
       }
       const int64_t k_end = std::min(E.k_cur + E.window, k_target[q]);
       E.P.k0 = E.k_cur;
       E.P.k_end = k_end;
       if (E.P.sample_mode == kSampleEq) E.P.t_end = std::min((double)k_end * E.P.delta, t_end[q]);
-      else E.P.t_end = std::min((double)(k_end - 1) * E.P.delta, t_end[q]);
+      else E.P.t_end = (double)(k_end - 1) * E.P.delta;  // flow output k_end-1 is at this time
+      if (k_end == k_target[q]) reached[q] = 1;
       act.push_back(ids[q]);
-      any = true;
     }
+    if (act.empty()) break;
     if ((rc = upload_ens(e))) return rc;
     for (int id : act) {
       Ens& E = e->ens[id];
       if (E.P.k_end > E.P.k0)
         HIPCHK(launch_fill_u32(e->d_stage + E.P.stage_off, 0u, (E.P.k_end - E.P.k0) * E.n * E.ncomp, e->st));
     }
-    rc = kmc_pass(e, act, mode, INT64_MAX, run_flag, &rs);
-    if (rc == DSM_ABORTED) { finish_stats(e, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
+    rc = kmc_pass(e, act, mode, false, run_flag, &rs);
+    if (rc == DSM_ABORTED) { finish_stats(e, ids, before, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
     if (rc) return rc;
     // consume the window's samples
     for (int id : act) {
@@ -526,10 +537,9 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
       const double frac = k_target[0] > 0 ? (double)E.k_cur / (double)k_target[0] : 1.0;
       *progress = (int32_t)(100.0 * frac);
     }
-    if (run_flag && *run_flag == 0) { finish_stats(e, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
-    if (!any) break;
+    if (run_flag && *run_flag == 0) { finish_stats(e, ids, before, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
   }
-  if ((rc = finish_stats(e, &rs, T))) return rc;
+  if ((rc = finish_stats(e, ids, before, &rs, T))) return rc;
   if (out) *out = rs;
   return DSM_OK;
 }
@@ -593,7 +603,7 @@ int dsm_destroy(dsm_engine* e) {
     if (E.dC) (void)hipFree(E.dC);
   }
   ChainArrays& c = e->ch;
-  void* ptrs[] = {c.Z, c.nk, c.ens, c.cap, c.stall, c.cid, c.local, c.base, c.t, c.ev, c.dt_prev, c.next_sample,
+  void* ptrs[] = {c.Z, c.nk, c.ens, c.cap, c.stall, c.cid, c.local, c.base, c.t, c.ev, c.dt_prev, c.next_sample, c.ev_stop, c.zsum,
                   c.Qx, c.Qy, c.Qz, c.N, c.w, e->d_ens, e->d_tabs, e->d_stage, e->d_work, e->d_cnt, e->d_counters,
                   e->d_corr_cnt, e->d_red, e->d_healthy};
   for (void* p : ptrs)
@@ -704,6 +714,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
     HIPCHK(hipMemcpy(e->ch.Z + c0, hzero.data(), 4 * E.n, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(e->ch.t + c0, 0, 8 * E.n));
     HIPCHK(hipMemset(e->ch.ev + c0, 0, 8 * E.n));
+    HIPCHK(hipMemset(e->ch.zsum + c0, 0, 8 * E.n));
     HIPCHK(hipMemset(e->ch.dt_prev + c0, 0, 4 * E.n));
     std::vector<int64_t> hns(E.n, E.k_cur);
     HIPCHK(hipMemcpy(e->ch.next_sample + c0, hns.data(), 8 * E.n, hipMemcpyHostToDevice));
@@ -867,7 +878,8 @@ int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mod
   if (rc) return rc;
   Timer T;
   dsm_run_stats rs{};
-  HIPCHK(launch_fill_u32(e->d_counters, 0u, 4, e->st));
+  unsigned long long before[2];
+  if ((rc = chain_totals(e, {ens}, before))) return rc;
   E->P.t_end = INFINITY;
   E->P.k0 = E->k_cur;
   E->P.k_end = E->k_cur;  // sampling suspended
@@ -881,7 +893,6 @@ int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mod
     a.n_work = e->d_cnt;
     a.overflow = e->d_work + (size_t)e->d_work_cap;
     a.n_overflow = e->d_cnt + kNumBins;
-    a.max_events = 1;
     HIPCHK(hipEventRecord(e->evs[0], e->st));
     for (int64_t k = 0; k < n_events; ++k) {
       HIPCHK(launch_kmc(a, S, true, E->n, e->st));
@@ -895,9 +906,10 @@ int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mod
     rs.first_kmc_ms = ms;
     rs.first_launches = rs.kmc_launches;
   } else {
-    if ((rc = kmc_pass(e, {ens}, mode, n_events, nullptr, &rs))) return rc;
+    HIPCHK(launch_set_ev_stop(e->ch.ev, e->ch.ev_stop, E->chain0, E->n, (uint64_t)n_events, e->st));
+    if ((rc = kmc_pass(e, {ens}, mode, true, nullptr, &rs))) return rc;
   }
-  if ((rc = finish_stats(e, &rs, T))) return rc;
+  if ((rc = finish_stats(e, {ens}, before, &rs, T))) return rc;
   if (stats) *stats = rs;
   return DSM_OK;
 }

@@ -35,8 +35,7 @@ __global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsP
   if (idx >= n) return;
   const int c = chains[idx];
   const EnsParams& P = ens[ch.ens[c]];
-  EnsLocal E;
-  load_ens(E, ens, ch.ens[c]);
+  const uint32_t k0 = P.key0, k1 = P.key1;
   const int nk = ch.nk[c];
   const uint32_t cid = ch.cid[c];
   const int64_t b = ch.base[c];
@@ -46,14 +45,14 @@ __global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsP
     ++len;
     if (i == nk) { ch.N[b + Z] = len; ++Z; break; }
     const int wi = i - 1;
-    if ((wi & 3) == 0) x = philox4x32_10(make_uint4((uint32_t)(wi >> 2), 0u, cid, kPurposeInitCut), E.key0, E.key1);
+    if ((wi & 3) == 0) x = philox4x32_10(make_uint4((uint32_t)(wi >> 2), 0u, cid, kPurposeInitCut), k0, k1);
     const uint32_t word = (wi & 3) == 0 ? x.x : (wi & 3) == 1 ? x.y : (wi & 3) == 2 ? x.z : x.w;
     if (word < P.cut_threshold) { ch.N[b + Z] = len; ++Z; len = 0; }
   }
   for (int k = 0; k < Z; ++k) {
     float qx = 0.0f, qy = 0.0f, qz = 0.0f;
     if (k > 0 && k < Z - 1) {
-      const uint4 g = philox4x32_10(make_uint4((uint32_t)k, 0u, cid, kPurposeInitQ), E.key0, E.key1);
+      const uint4 g = philox4x32_10(make_uint4((uint32_t)k, 0u, cid, kPurposeInitQ), k0, k1);
       float g0, g1, g2, g3;
       boxmuller(g.x, g.y, g0, g1);
       boxmuller(g.z, g.w, g2, g3);
@@ -63,14 +62,14 @@ __global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsP
     ch.Qx[b + k] = qx; ch.Qy[b + k] = qy; ch.Qz[b + k] = qz;
     ch.w[b + k] = 0.0f;
   }
-  if (E.cd) {
+  if (P.cd) {
     for (int k = 0; k < Z - 1; ++k) {
-      if ((k & 3) == 0) x = philox4x32_10(make_uint4((uint32_t)(k >> 2), 0u, cid, kPurposeInitTau), E.key0, E.key1);
+      if ((k & 3) == 0) x = philox4x32_10(make_uint4((uint32_t)(k >> 2), 0u, cid, kPurposeInitTau), k0, k1);
       const uint32_t word = (k & 3) == 0 ? x.x : (k & 3) == 1 ? x.y : (k & 3) == 2 ? x.z : x.w;
-      ch.w[b + k] = rate_ft(E, u01(word));
+      ch.w[b + k] = rate_ft(&P, u01(word));
     }
   }
-  ch.Z[c] = Z; ch.stall[c] = 0; ch.t[c] = 0.0; ch.ev[c] = 0; ch.dt_prev[c] = 0.0f;
+  ch.Z[c] = Z; ch.stall[c] = 0; ch.t[c] = 0.0; ch.ev[c] = 0; ch.dt_prev[c] = 0.0f; ch.zsum[c] = 0;
   ch.next_sample[c] = (P.sample_mode == kSampleFlow) ? 1 : 0;
 }
 
@@ -78,6 +77,27 @@ __global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsP
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// one block: out[0] += sum of ev, out[1] += sum of zsum over a chain range
+__global__ void __launch_bounds__(1024) sum_counters_kernel(const uint64_t* ev, const uint64_t* zsum, int chain0,
+                                                             int n, unsigned long long* out) {
+  __shared__ unsigned long long r0[1024], r1[1024];
+  unsigned long long a0 = 0, a1 = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) { a0 += ev[chain0 + i]; a1 += zsum[chain0 + i]; }
+  r0[threadIdx.x] = a0;
+  r1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) { r0[threadIdx.x] += r0[threadIdx.x + h]; r1[threadIdx.x] += r1[threadIdx.x + h]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] += r0[0]; out[1] += r1[0]; }
+}
+
+__global__ void set_ev_stop_kernel(const uint64_t* ev, uint64_t* ev_stop, int chain0, int n, uint64_t add) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ev_stop[chain0 + i] = ev[chain0 + i] + add;
 }
 
 __global__ void iota_kernel(int* p, int start, int n) {
@@ -258,13 +278,13 @@ hipError_t launch_kmc(const KmcArgs& a, int S, bool plain, int grid, hipStream_t
   }
 }
 
-int kmc_max_blocks_per_cu(int S, bool plain) {
+int kmc_max_blocks_per_cu(int S, bool plain, int lds_floats) {
   switch (S) {
-    case 1: return kmc_occupancy_slots_1(plain);
-    case 2: return kmc_occupancy_slots_2(plain);
-    case 4: return kmc_occupancy_slots_4(plain);
-    case 8: return kmc_occupancy_slots_8(plain);
-    case 16: return kmc_occupancy_slots_16(plain);
+    case 1: return kmc_occupancy_slots_1(plain, lds_floats);
+    case 2: return kmc_occupancy_slots_2(plain, lds_floats);
+    case 4: return kmc_occupancy_slots_4(plain, lds_floats);
+    case 8: return kmc_occupancy_slots_8(plain, lds_floats);
+    case 16: return kmc_occupancy_slots_16(plain, lds_floats);
     default: return 0;
   }
 }
@@ -278,6 +298,20 @@ hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int
 hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (uint32_t*)p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_ev_stop(const uint64_t* ev, uint64_t* ev_stop, int chain0, int n, uint64_t add,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(set_ev_stop_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ev, ev_stop, chain0, n, add);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_counters(const uint64_t* ev, const uint64_t* zsum, int chain0, int n, unsigned long long* out,
+                               hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sum_counters_kernel, dim3(1), dim3(1024), 0, st, ev, zsum, chain0, n, out);
   return hipGetLastError();
 }
 

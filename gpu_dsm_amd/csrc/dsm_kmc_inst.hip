@@ -18,15 +18,17 @@ hipError_t DSM_CAT(launch_kmc_slots_, DSM_SLOTS)(const KmcArgs& a, bool plain, i
     const int blocks = (int)(((int64_t)grid * 64 + 255) / 256);
     hipLaunchKernelGGL(kmc_plain_kernel<DSM_SLOTS>, dim3(blocks), dim3(256), 0, st, a);
   } else {
-    hipLaunchKernelGGL(kmc_batched_kernel<DSM_SLOTS>, dim3(grid), dim3(256), 0, st, a);
+    const size_t lds = (size_t)4 * a.lds_floats * sizeof(float);
+    hipLaunchKernelGGL(kmc_batched_kernel<DSM_SLOTS>, dim3(grid), dim3(256), lds, st, a);
   }
   return hipGetLastError();
 }
 
-int DSM_CAT(kmc_occupancy_slots_, DSM_SLOTS)(bool plain) {
+int DSM_CAT(kmc_occupancy_slots_, DSM_SLOTS)(bool plain, int lds_floats) {
   int nb = 0;
+  const size_t lds = (size_t)4 * lds_floats * sizeof(float);
   const hipError_t e = plain ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_plain_kernel<DSM_SLOTS>, 256, 0)
-                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_batched_kernel<DSM_SLOTS>, 256, 0);
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_batched_kernel<DSM_SLOTS>, 256, lds);
   return e == hipSuccess ? nb : 0;
 }
 

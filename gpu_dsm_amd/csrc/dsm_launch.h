@@ -12,17 +12,22 @@ namespace dsm {
 hipError_t launch_kmc(const KmcArgs& a, int S, bool plain, int grid, hipStream_t st);
 #define DSM_DECLARE_SLOTS(S)                                                                   \
   hipError_t launch_kmc_slots_##S(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
-  int kmc_occupancy_slots_##S(bool plain);
+  int kmc_occupancy_slots_##S(bool plain, int lds_floats);
 DSM_DECLARE_SLOTS(1)
 DSM_DECLARE_SLOTS(2)
 DSM_DECLARE_SLOTS(4)
 DSM_DECLARE_SLOTS(8)
 DSM_DECLARE_SLOTS(16)
 #undef DSM_DECLARE_SLOTS
-int kmc_max_blocks_per_cu(int S, bool plain);
+int kmc_max_blocks_per_cu(int S, bool plain, int lds_floats);
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int* chains, int n, hipStream_t st);
 hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_iota(int* p, int start, int n, hipStream_t st);
+// out[0] += sum ev[chain0..chain0+n), out[1] += sum zsum[...]
+hipError_t launch_sum_counters(const uint64_t* ev, const uint64_t* zsum, int chain0, int n, unsigned long long* out,
+                               hipStream_t st);
+hipError_t launch_set_ev_stop(const uint64_t* ev, uint64_t* ev_stop, int chain0, int n, uint64_t add,
+                              hipStream_t st);
 hipError_t launch_flow_reduce(const float* stage, const int* stall, int chain0, int nch, int nrows, double* out,
                               hipStream_t st);
 hipError_t launch_corr_update(const float* stage, double* D, double* A, double* C, const int* stall, int chain0,

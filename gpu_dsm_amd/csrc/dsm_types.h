@@ -3,7 +3,7 @@
 // Chain state lives in HBM as structure-of-arrays.  Per chain (length n_chains):
 //   Z, nk, ens, cap, stall, cid (Philox chain id within its ensemble),
 //   local (index within its ensemble), base (offset of strand 0),
-//   t (FP64 clock), ev (event counter), dt_prev (FP32, flow), next_sample.
+//   t (FP64 clock), ev (event counter), dt_prev (FP32, flow), next_sample, ev_stop, zsum.
 // Per strand slot (length total capacity, chain c owns [base[c], base[c]+cap[c])):
 //   Qx, Qy, Qz (FP32), N (int32), w (FP32 CD destruction rate of the link to
 //   the right of the strand).
@@ -55,6 +55,8 @@ struct ChainArrays {
   uint64_t* ev;
   float* dt_prev;
   int64_t* next_sample;
+  uint64_t* ev_stop;  // jump-count target of the current fixed-count pass
+  uint64_t* zsum;     // sum over the chain's jumps of Z at the jump (byte model)
   float* Qx;
   float* Qy;
   float* Qz;
@@ -72,8 +74,8 @@ struct KmcArgs {
   int* overflow;        // chains that outgrew this kernel's slot capacity
   int* n_overflow;
   float* stage;         // sample staging arena
-  int64_t max_events;   // per chain per launch
-  unsigned long long* counters;  // [0] events, [1] sum of Z over events
+  int use_ev_stop;      // stop each chain when ev reaches ch.ev_stop[c]
+  int lds_floats;       // per-wave LDS table copy size (0: tables read from global memory)
 };
 
 }  // namespace dsm
