@@ -53,21 +53,24 @@ def traffic_from_profile():
 
 
 def cpu_baseline(args):
+    """Time the oracle on the first chains of the same ensemble for about args.cpu_seconds."""
     from oracle import oracle as O
     w = WORKLOAD
     P = O.Params(w["beta"], w["nk"], w["cd"])
     threads = min(os.cpu_count() or 1, 16)
-    # calibrate the subset size to roughly args.cpu_seconds of work
-    n, t_end = 256, 20.0
-    ev, sec = O.bench(P, args.seed, n, t_end, threads)
+    # calibration: 512 chains to t = 20 tau_K
+    n_cal, t_cal = 512, 20.0
+    ev, sec = O.bench(P, args.seed, n_cal, t_cal, threads)
     rate = ev / max(sec, 1e-9)
-    n_target = int(min(16384, max(256, n * args.cpu_seconds / max(sec, 1e-3) / 1.2)))
-    ev, sec = O.bench(P, args.seed, n_target, t_end, threads)
-    rate = ev / sec
-    return {"value": rate, "unit": "jumps/s", "cores": threads, "kind": "port",
-            "sample": "first %d chains of the same ensemble (same Philox streams) advanced to t=%g tau_K "
-                      "with the mirror-FP32 oracle on %d host threads: %d jumps in %.1f s"
-                      % (n_target, t_end, threads, ev, sec)}
+    per_chain_tau = ev / (n_cal * t_cal)
+    n = min(args.chains, 16384)
+    t_end = args.cpu_seconds * rate / (n * per_chain_tau)
+    t_end = float(max(20.0, min(1.0e4, round(t_end))))
+    ev, sec = O.bench(P, args.seed, n, t_end, threads)
+    return {"value": ev / sec, "unit": "jumps/s", "cores": threads, "kind": "port",
+            "sample": "first %d chains of the same ensemble (same Philox streams) advanced from equilibrium to "
+                      "t=%g tau_K with the mirror-FP32 oracle on %d host threads: %d jumps in %.1f s"
+                      % (n, t_end, threads, ev, sec)}
 
 
 def main():
@@ -116,6 +119,9 @@ def main():
     plain_gbs = plain_bytes / (pst["kmc_ms"] * 1e-3) / 1e9 if pst["kmc_ms"] > 0 else 0.0
     lag, gt = eng.correlator(ens)
     eng.close()
+    copy_bytes = 1 << 30
+    copy_ms = g.probe_copy(copy_bytes, 20)
+    copy_gbs = 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -157,6 +163,8 @@ def main():
                   "device_ms": pst["kmc_ms"]},
         "batched_over_plain": value / plain_rate if plain_rate > 0 else None,
         "G0": float(gt[0]) if len(gt) else None,
+        "copy_probe": {"bytes": copy_bytes, "ms": copy_ms, "GBs": copy_gbs,
+                       "note": "plain 16-byte device copy, read+write bytes over time: achievable HBM rate"},
     }
     print(json.dumps(out))
 

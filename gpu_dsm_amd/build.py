@@ -26,7 +26,7 @@ COMMON = [
 ]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["dsm_kernels.hip", "dsm_engine.cpp", "dsm_driver.cpp"]
-SLOTS = [1, 2, 4, 8, 16]  # register slots per lane of the KMC kernels (one translation unit each)
+GEOMS = [0, 1, 2, 3]  # KMC chain geometries (one translation unit each)
 
 
 def hipcc():
@@ -47,9 +47,9 @@ def _run(cmd):
 def build(verbose=False, jobs=None):
     os.makedirs(OUT, exist_ok=True)
     units = [(os.path.join(CSRC, s), os.path.join(OUT, os.path.splitext(s)[0] + ".o"), []) for s in SOURCES]
-    units += [(os.path.join(CSRC, "dsm_kmc_inst.hip"), os.path.join(OUT, "dsm_kmc_s%d.o" % s), ["-DDSM_SLOTS=%d" % s])
-              for s in SLOTS]
-    # largest slot counts first: they dominate the compile time
+    units += [(os.path.join(CSRC, "dsm_kmc_inst.hip"), os.path.join(OUT, "dsm_kmc_g%d.o" % g), ["-DDSM_GEOM=%d" % g])
+              for g in GEOMS]
+    # largest geometries first: they dominate the compile time
     units.sort(key=lambda u: -int(u[2][0].split("=")[1]) if u[2] else 0)
     jobs = jobs or max(1, min(len(units), os.cpu_count() or 1, 16))
     pending, running, objs = list(units), [], []

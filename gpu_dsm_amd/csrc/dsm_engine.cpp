@@ -38,8 +38,7 @@ int fail(int code, const char* fmt, ...) {
     if (_e != hipSuccess) return fail(DSM_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(_e)); \
   } while (0)
 
-constexpr int kSlots[5] = {1, 2, 4, 8, 16};
-constexpr int kNumBins = 5;
+constexpr int kNumBins = kNumGeoms;  // work bins = KMC chain geometries (capacity 64, 256, 512, 1024)
 constexpr int kDefaultWindow = 256;
 constexpr int kLdsTableFloats = 2816;  // per-wave LDS budget for rate tables (Nk <= ~200)
 
@@ -106,11 +105,14 @@ void build_tables(std::vector<float>& t, int nmax, float beta, int cd, float c_c
   for (int x = 1; x < fr; ++x) FR[x] = 4.0f / (float)x;
 }
 
-int slots_for(int nmax) {  // smallest S with 64*S > nmax (room for one insertion)
+int bin_for(int nmax) {  // smallest geometry with capacity > nmax (room for one insertion)
   for (int b = 0; b < kNumBins; ++b)
-    if (64 * kSlots[b] > nmax) return b;
+    if (kmc_geom_capacity(b) > nmax) return b;
   return -1;
 }
+
+// strand slots per chain: Nk rounded up to whole 16-byte groups (aligned vector access)
+inline int cap_of(int nk) { return (nk + 3) & ~3; }
 
 template <typename T>
 int dev_alloc(T** p, size_t n) {
@@ -128,7 +130,7 @@ struct Ens {
   int chain0 = 0, n = 0, nmax = 0;
   int64_t strand0 = 0, nstrand = 0;
   EnsParams P{};
-  int bin_max = 0;      // largest bin (slot count index) a chain of this ensemble can need
+  int bin_max = 0;      // largest geometry bin a chain of this ensemble can need
   // sampling
   int ncomp = 0;        // floats per sample per chain (3 G(t), 8 flow)
   int window = 0;
@@ -170,7 +172,7 @@ struct dsm_engine {
   // work lists: [bin][nch] + counts
   int* d_work = nullptr;
   int d_work_cap = 0;
-  int* d_cnt = nullptr;     // [kNumBins] list counts, [kNumBins..2k) queues
+  int* d_cnt = nullptr;     // [kNumBins + 1] list counts, then [kNumBins + 1] queues
   unsigned long long* d_counters = nullptr;
   int* d_corr_cnt = nullptr;
   int d_corr_cnt_cap = 0;
@@ -217,10 +219,12 @@ int grow_chains(dsm_engine* e, int n_new, int64_t s_new) {
       (rc = grow(&c.Qz, s0, s_new, e->st)) || (rc = grow(&c.N, s0, s_new, e->st)) ||
       (rc = grow(&c.w, s0, s_new, e->st)))
     return rc;
-  if (n_new > e->d_work_cap) {
+  // bin 0 pads every ensemble to whole quads; one spare list past the last bin
+  const int need = n_new + 4 * ((int)e->ens.size() + 2);
+  if (need > e->d_work_cap) {
     if (e->d_work) HIPCHK(hipFree(e->d_work));
-    if ((rc = dev_alloc(&e->d_work, (size_t)kNumBins * n_new))) return rc;
-    e->d_work_cap = n_new;
+    if ((rc = dev_alloc(&e->d_work, (size_t)(kNumBins + 1) * need))) return rc;
+    e->d_work_cap = need;
   }
   e->nch = n_new;
   e->nstr = s_new;
@@ -323,8 +327,8 @@ struct Timer {
   }
 };
 
-int device_blocks(dsm_engine* e, int S, bool plain, int lds_floats) {
-  int nb = kmc_max_blocks_per_cu(S, plain, lds_floats);
+int device_blocks(dsm_engine* e, int geom, bool plain, int lds_floats) {
+  int nb = kmc_max_blocks_per_cu(geom, plain, lds_floats);
   if (nb <= 0) nb = 1;
   return nb * e->n_cu;
 }
@@ -350,48 +354,88 @@ int chain_totals(dsm_engine* e, const std::vector<int>& ids, unsigned long long 
   return DSM_OK;
 }
 
-// One KMC pass over every chain of the given ensembles until each reaches its
-// ensemble's P.t_end (batched: persistent Z-binned launches; plain: one jump per
-// launch until no chain moves).  P must already be uploaded.
-int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_stop,
-             volatile const int32_t* run_flag, dsm_run_stats* rs) {
-  // work list = all chains of the ensembles, bin 0 gets everything, kernels push overflow up
-  int total = 0, bin_max = 0;
-  for (int id : ids) { total += e->ens[id].n; bin_max = std::max(bin_max, e->ens[id].bin_max); }
-  if (total == 0) return DSM_OK;
-  int* cnt = e->d_cnt;
-  int* list0 = e->d_work;
+// Work of one KMC pass over the given ensembles: the bin-0 list holds every chain,
+// each ensemble padded with -1 to whole quads (the row form runs four chains of one
+// ensemble per wave); the wave-form bins receive the chains that outgrow a geometry.
+struct Pass {
+  int n0 = 0;  // entries of the bin-0 list
+  int bin_max = 0;
+  KmcArgs a{};
+};
+
+int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pass& ps) {
+  int n0 = 0, bin_max = 0, tab_max = 0;
+  for (int id : ids) {
+    n0 += (e->ens[id].n + 3) & ~3;
+    bin_max = std::max(bin_max, e->ens[id].bin_max);
+    tab_max = std::max(tab_max, 9 * (e->ens[id].nmax + 2) + 4 * e->ens[id].nmax + 4);
+  }
+  ps.n0 = n0;
+  ps.bin_max = bin_max;
+  if (n0 == 0) return DSM_OK;
+  HIPCHK(launch_fill_u32(e->d_work, 0xFFFFFFFFu, n0, e->st));
   int off = 0;
   for (int id : ids) {
-    int rc = (int)launch_iota(list0 + off, e->ens[id].chain0, e->ens[id].n, e->st);
-    if (rc) return fail(DSM_ERR_DEVICE, "iota launch failed");
-    off += e->ens[id].n;
+    HIPCHK(launch_iota(e->d_work + off, e->ens[id].chain0, e->ens[id].n, e->st));
+    off += (e->ens[id].n + 3) & ~3;
   }
-  KmcArgs a = base_args(e);
-  a.use_ev_stop = use_ev_stop ? 1 : 0;
+  ps.a = base_args(e);
+  ps.a.use_ev_stop = use_ev_stop ? 1 : 0;
+  // rate tables go to LDS (one copy per wave) when they fit the per-wave budget
+  ps.a.lds_floats = tab_max <= kLdsTableFloats ? tab_max : 0;
+  return DSM_OK;
+}
+
+// One pass through the geometry bins.  Batched: each chain runs to its targets;
+// plain: each chain makes one jump.  Times every launch when `timed`.
+int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs) {
+  int* cnt = e->d_cnt;
+  HIPCHK(launch_fill_u32(cnt, 0u, 2 * (kNumBins + 1), e->st));
+  HIPCHK(launch_fill_u32(cnt, (uint32_t)ps.n0, 1, e->st));
+  KmcArgs a = ps.a;
+  for (int b = 0; b <= ps.bin_max; ++b) {
+    a.work = e->d_work + (size_t)b * e->d_work_cap;
+    a.n_work = cnt + b;
+    a.queue = cnt + (kNumBins + 1) + b;
+    a.overflow = e->d_work + (size_t)(b + 1) * e->d_work_cap;
+    a.n_overflow = cnt + b + 1;
+    const int grid = device_blocks(e, b, plain, a.lds_floats);
+    if (timed) HIPCHK(hipEventRecord(e->evs[0], e->st));
+    HIPCHK(launch_kmc(a, b, plain, grid, e->st));
+    rs->kmc_launches++;
+    if (b == 0) rs->first_launches++;
+    if (timed) {
+      HIPCHK(hipEventRecord(e->evs[1], e->st));
+      HIPCHK(hipEventSynchronize(e->evs[1]));
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+      rs->kmc_ms += ms;
+      if (b == 0) rs->first_kmc_ms += ms;
+    }
+  }
+  return DSM_OK;
+}
+
+// One KMC pass over every chain of the given ensembles until each reaches its
+// ensemble's P.t_end (batched: persistent geometry-binned launches; plain: one jump
+// per chain per pass until no chain moves).  P must already be uploaded.
+int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_stop,
+             volatile const int32_t* run_flag, dsm_run_stats* rs) {
+  Pass ps;
+  int rc = pass_setup(e, ids, use_ev_stop, ps);
+  if (rc || ps.n0 == 0) return rc;
   if (mode == DSM_MODE_PLAIN) {
-    // plain: every chain in the slot count of its ensemble's longest possible chain
-    const int S = kSlots[bin_max];
-    HIPCHK(launch_fill_u32(cnt, (uint32_t)total, 1, e->st));
-    a.work = list0;
-    a.n_work = cnt;
-    a.overflow = e->d_work + (size_t)e->d_work_cap;  // never used at the maximal slot count
-    a.n_overflow = cnt + kNumBins;
     const int batch = 16;
     for (;;) {
       unsigned long long before[2], after[2];
-      int rc = chain_totals(e, ids, before);
-      if (rc) return rc;
-      HIPCHK(hipEventRecord(e->evs[0], e->st));
-      for (int b = 0; b < batch; ++b) {
-        HIPCHK(launch_kmc(a, S, true, total, e->st));
-        rs->kmc_launches++;
-        rs->first_launches++;
-      }
-      HIPCHK(hipEventRecord(e->evs[1], e->st));
+      if ((rc = chain_totals(e, ids, before))) return rc;
+      HIPCHK(hipEventRecord(e->evs[2], e->st));
+      for (int k = 0; k < batch; ++k)
+        if ((rc = pass_bins(e, ps, true, false, rs))) return rc;
+      HIPCHK(hipEventRecord(e->evs[3], e->st));
       if ((rc = chain_totals(e, ids, after))) return rc;
       float ms = 0.0f;
-      HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+      HIPCHK(hipEventElapsedTime(&ms, e->evs[2], e->evs[3]));
       rs->kmc_ms += ms;
       rs->first_kmc_ms += ms;
       if (after[0] == before[0]) break;
@@ -399,31 +443,7 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_s
     }
     return DSM_OK;
   }
-  // batched; tables go to LDS (one copy per wave) when they fit the per-wave budget
-  int tab_max = 0;
-  for (int id : ids) tab_max = std::max(tab_max, 9 * (e->ens[id].nmax + 2) + 4 * e->ens[id].nmax + 4);
-  a.lds_floats = tab_max <= kLdsTableFloats ? tab_max : 0;
-  HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumBins, e->st));
-  HIPCHK(launch_fill_u32(cnt, (uint32_t)total, 1, e->st));
-  for (int b = 0; b <= bin_max; ++b) {
-    a.work = e->d_work + (size_t)b * e->d_work_cap;
-    a.n_work = cnt + b;
-    a.queue = cnt + kNumBins + b;
-    const int nb = std::min(b + 1, kNumBins - 1);
-    a.overflow = e->d_work + (size_t)nb * e->d_work_cap;
-    a.n_overflow = cnt + nb;
-    const int S = kSlots[b];
-    const int grid = device_blocks(e, S, false, a.lds_floats);
-    HIPCHK(hipEventRecord(e->evs[0], e->st));
-    HIPCHK(launch_kmc(a, S, false, grid, e->st));
-    HIPCHK(hipEventRecord(e->evs[1], e->st));
-    HIPCHK(hipEventSynchronize(e->evs[1]));
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
-    rs->kmc_ms += ms;
-    rs->kmc_launches++;
-    if (b == 0) { rs->first_kmc_ms += ms; rs->first_launches++; }
-  }
+  if ((rc = pass_bins(e, ps, false, true, rs))) return rc;
   if (run_flag && *run_flag == 0) return DSM_ABORTED;
   return DSM_OK;
 }
@@ -578,7 +598,7 @@ int dsm_create(int32_t device_id, dsm_engine** out) {
   }
   for (auto& ev : e->evs) HIPCHK(hipEventCreate(&ev));
   int rc;
-  if ((rc = dev_alloc(&e->d_cnt, 4 * kNumBins)) || (rc = dev_alloc(&e->d_counters, 4)) ||
+  if ((rc = dev_alloc(&e->d_cnt, 4 * (kNumBins + 1))) || (rc = dev_alloc(&e->d_counters, 4)) ||
       (rc = dev_alloc(&e->d_healthy, 1))) {
     dsm_destroy(e);
     return rc;
@@ -635,7 +655,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
     nmax = std::max(nmax, nk);
   }
   E.nmax = nmax;
-  E.bin_max = slots_for(nmax);
+  E.bin_max = bin_for(nmax);
   if (E.bin_max < 0) return fail(DSM_ERR_ARG, "Nk too large");
   // parameters
   const int nk_ref = d->nk_per_chain ? nmax : d->nk;
@@ -691,7 +711,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   E.chain0 = e->nch;
   E.strand0 = e->nstr;
   E.nstrand = 0;
-  for (int i = 0; i < E.n; ++i) E.nstrand += E.nk[i];
+  for (int i = 0; i < E.n; ++i) E.nstrand += cap_of(E.nk[i]);
   P.chain0 = E.chain0;
   if ((rc = grow_chains(e, e->nch + E.n, e->nstr + E.nstrand))) return rc;
   {
@@ -701,7 +721,8 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
     std::vector<int64_t> hbase(E.n);
     int64_t b = E.strand0;
     for (int i = 0; i < E.n; ++i) {
-      hnk[i] = E.nk[i]; hcap[i] = E.nk[i]; hcid[i] = (uint32_t)i; hloc[i] = i; hbase[i] = b; b += E.nk[i];
+      hnk[i] = E.nk[i]; hcap[i] = cap_of(E.nk[i]); hcid[i] = (uint32_t)i; hloc[i] = i; hbase[i] = b;
+      b += cap_of(E.nk[i]);
     }
     const int c0 = E.chain0;
     HIPCHK(hipMemcpy(e->ch.nk + c0, hnk.data(), 4 * E.n, hipMemcpyHostToDevice));
@@ -795,7 +816,7 @@ int dsm_get_state(dsm_engine* e, int32_t ens, int32_t cap, int32_t* Z, int32_t* 
         }
         if (w) w[o] = (k < hZ[i] - 1) ? ww[b + k] : 0.0f;
       }
-      b += E->nk[i];
+      b += cap_of(E->nk[i]);
     }
   }
   return DSM_OK;
@@ -830,7 +851,7 @@ int dsm_set_state(dsm_engine* e, int32_t ens, int32_t cap, const int32_t* Z, con
       nn[b + k] = N[o]; qx[b + k] = Q[o * 3 + 0]; qy[b + k] = Q[o * 3 + 1]; qz[b + k] = Q[o * 3 + 2];
       ww[b + k] = (k < Z[i] - 1) ? w[o] : 0.0f;
     }
-    b += E->nk[i];
+    b += cap_of(E->nk[i]);
   }
   HIPCHK(hipStreamSynchronize(e->st));
   HIPCHK(hipMemcpy(e->ch.Qx + s0, qx.data(), 4 * ns, hipMemcpyHostToDevice));
@@ -885,26 +906,17 @@ int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mod
   E->P.k_end = E->k_cur;  // sampling suspended
   if ((rc = upload_ens(e))) return rc;
   if (mode == DSM_MODE_PLAIN) {
-    const int S = kSlots[E->bin_max];
-    HIPCHK(launch_iota(e->d_work, E->chain0, E->n, e->st));
-    HIPCHK(launch_fill_u32(e->d_cnt, (uint32_t)E->n, 1, e->st));
-    KmcArgs a = base_args(e);
-    a.work = e->d_work;
-    a.n_work = e->d_cnt;
-    a.overflow = e->d_work + (size_t)e->d_work_cap;
-    a.n_overflow = e->d_cnt + kNumBins;
-    HIPCHK(hipEventRecord(e->evs[0], e->st));
-    for (int64_t k = 0; k < n_events; ++k) {
-      HIPCHK(launch_kmc(a, S, true, E->n, e->st));
-      rs.kmc_launches++;
-    }
-    HIPCHK(hipEventRecord(e->evs[1], e->st));
-    HIPCHK(hipEventSynchronize(e->evs[1]));
+    Pass ps;
+    if ((rc = pass_setup(e, {ens}, false, ps))) return rc;
+    HIPCHK(hipEventRecord(e->evs[2], e->st));
+    for (int64_t k = 0; k < n_events; ++k)
+      if ((rc = pass_bins(e, ps, true, false, &rs))) return rc;
+    HIPCHK(hipEventRecord(e->evs[3], e->st));
+    HIPCHK(hipEventSynchronize(e->evs[3]));
     float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+    HIPCHK(hipEventElapsedTime(&ms, e->evs[2], e->evs[3]));
     rs.kmc_ms = ms;
     rs.first_kmc_ms = ms;
-    rs.first_launches = rs.kmc_launches;
   } else {
     HIPCHK(launch_set_ev_stop(e->ch.ev, e->ch.ev_stop, E->chain0, E->n, (uint64_t)n_events, e->st));
     if ((rc = kmc_pass(e, {ens}, mode, true, nullptr, &rs))) return rc;
@@ -1013,8 +1025,8 @@ static int probe_run(int32_t device_id, const void* in, size_t in_bytes, void* o
   return DSM_OK;
 }
 
-static hipError_t l_scan(const void* i, void* o, int n, int, hipStream_t s) {
-  return launch_scan_probe((const float*)i, (float*)o, n, s);
+static hipError_t l_scan(const void* i, void* o, int n, int form, hipStream_t s) {
+  return launch_scan_probe((const float*)i, (float*)o, n, form, s);
 }
 static hipError_t l_math(const void* i, void* o, int n, int w, hipStream_t s) {
   return launch_math_probe((const float*)i, (float*)o, n, w, s);
@@ -1023,15 +1035,11 @@ static hipError_t l_philox(const void* i, void* o, int n, int, hipStream_t s) {
   return launch_philox_probe((const uint32_t*)i, (uint32_t*)o, n, s);
 }
 
-int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n) {
-  if (n <= 0 || !in || !out) return fail(DSM_ERR_ARG, "bad probe arguments");
-  const int np = (n + 63) / 64 * 64;
-  std::vector<float> buf(np, 0.0f), res(np, 0.0f);
-  std::memcpy(buf.data(), in, 4 * n);
-  int rc = probe_run(device_id, buf.data(), 4 * np, res.data(), 4 * np, l_scan, n, 0);
-  if (rc) return rc;
-  std::memcpy(out, res.data(), 4 * n);
-  return DSM_OK;
+int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form) {
+  if (!in || !out || (form != 0 && form != 1) || n <= 0 || n > (form == 1 ? 1024 : 64))
+    return fail(DSM_ERR_ARG, "bad probe arguments");
+  const int nin = form == 1 ? n : 4 * n, nout = form == 1 ? n + 1 : 4 * (n + 1);
+  return probe_run(device_id, in, 4 * (size_t)nin, out, 4 * (size_t)nout, l_scan, n, form);
 }
 int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which) {
   if (n <= 0 || !in || !out || which < 0 || which > 3) return fail(DSM_ERR_ARG, "bad probe arguments");
@@ -1040,6 +1048,38 @@ int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, in
 int dsm_probe_philox(int32_t device_id, const uint32_t* ck, uint32_t* out, int32_t n) {
   if (n <= 0 || !ck || !out) return fail(DSM_ERR_ARG, "bad probe arguments");
   return probe_run(device_id, ck, 24 * (size_t)n, out, 16 * (size_t)n, l_philox, n, 0);
+}
+
+int dsm_probe_copy(int32_t device_id, int64_t n_bytes, int32_t reps, double* ms_per_copy) {
+  if (n_bytes < 16 || (n_bytes & 15) || reps <= 0 || !ms_per_copy) return fail(DSM_ERR_ARG, "bad probe arguments");
+  HIPCHK(hipSetDevice(device_id));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device_id));
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, (size_t)n_bytes));
+  if (hipMalloc(&b, (size_t)n_bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(DSM_ERR_MEMORY, "copy probe: out of device memory");
+  }
+  const size_t n4 = (size_t)n_bytes / 16;
+  const int grid = prop.multiProcessorCount * 16;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipMemset(a, 0, (size_t)n_bytes));
+  HIPCHK(launch_copy_probe(a, b, n4, grid, nullptr));  // warm-up
+  HIPCHK(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; ++r) HIPCHK(launch_copy_probe(a, b, n4, grid, nullptr));
+  HIPCHK(hipEventRecord(e1, nullptr));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = (double)ms / reps;
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  HIPCHK(hipFree(a));
+  HIPCHK(hipFree(b));
+  return DSM_OK;
 }
 
 }  // extern "C"

@@ -230,14 +230,52 @@ __global__ void __launch_bounds__(256) corr_reduce_kernel(const double* C, const
   if (threadIdx.x == 0) { out_sum[pt] = red[0]; if (pt == 0) *out_healthy = cnt[0]; }
 }
 
+// ------------------------------------------------------------------ copy probe (bandwidth)
+// Plain 16-byte copy, grid-stride: the reference point for HBM byte counters and the
+// achievable copy bandwidth of the device.
+__global__ void copy_probe_kernel(const float4* __restrict__ in, float4* __restrict__ out, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) out[i] = in[i];
+}
+
 // ------------------------------------------------------------------ scan probe (tests)
-__global__ void scan_probe_kernel(const float* in, float* out, int n) {
-  const int s = blockIdx.x;
-  if (s * 64 >= n) return;
-  const int i = s * 64 + (int)threadIdx.x;
-  const float v = i < n ? in[i] : 0.0f;
-  const float r = wave_scan(v);
-  if (i < n) out[i] = r;
+// Canonical scan tree of the KMC kernels.  form 1: one chain of n <= 1024 positions
+// (wave form, four register groups): out[0..n) inclusive sums, out[n] total.
+// form 0: four chains of n <= 64 positions (row form): chain r reads in[r*n ..] and
+// writes out[r*(n+1) ..].
+__global__ void __launch_bounds__(64) scan_probe_kernel(const float* in, float* out, int n, int form) {
+  if (form == 1) {
+    const LaneC<1> L;
+    float v[16], g[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int p = L.pos(j);
+      v[j] = p < n ? in[p] : 0.0f;
+    }
+    const float tot = chain_scan<1, 4>(v, g, n, L);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int p = L.pos(j);
+      if (p < n) out[p] = g[j];
+    }
+    if (L.lane == 0) out[n] = tot;
+  } else {
+    const LaneC<4> L;
+    const int r = L.lane >> 4;
+    float v[4], g[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = L.pos(j);
+      v[j] = p < n ? in[r * n + p] : 0.0f;
+    }
+    const float tot = chain_scan<4, 1>(v, g, n, L);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = L.pos(j);
+      if (p < n) out[r * (n + 1) + p] = g[j];
+    }
+    if (L.sl == 0) out[r * (n + 1) + n] = tot;
+  }
 }
 
 __global__ void math_probe_kernel(const float* in, float* out, int n, int which) {
@@ -267,27 +305,27 @@ __global__ void philox_probe_kernel(const uint32_t* ctr_key, uint32_t* out, int 
 
 namespace dsm {
 
-hipError_t launch_kmc(const KmcArgs& a, int S, bool plain, int grid, hipStream_t st) {
-  switch (S) {
-    case 1: return launch_kmc_slots_1(a, plain, grid, st);
-    case 2: return launch_kmc_slots_2(a, plain, grid, st);
-    case 4: return launch_kmc_slots_4(a, plain, grid, st);
-    case 8: return launch_kmc_slots_8(a, plain, grid, st);
-    case 16: return launch_kmc_slots_16(a, plain, grid, st);
+hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st) {
+  switch (geom) {
+    case 0: return launch_kmc_geom_0(a, plain, grid, st);
+    case 1: return launch_kmc_geom_1(a, plain, grid, st);
+    case 2: return launch_kmc_geom_2(a, plain, grid, st);
+    case 3: return launch_kmc_geom_3(a, plain, grid, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-int kmc_max_blocks_per_cu(int S, bool plain, int lds_floats) {
-  switch (S) {
-    case 1: return kmc_occupancy_slots_1(plain, lds_floats);
-    case 2: return kmc_occupancy_slots_2(plain, lds_floats);
-    case 4: return kmc_occupancy_slots_4(plain, lds_floats);
-    case 8: return kmc_occupancy_slots_8(plain, lds_floats);
-    case 16: return kmc_occupancy_slots_16(plain, lds_floats);
+int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats) {
+  switch (geom) {
+    case 0: return kmc_occupancy_geom_0(plain, lds_floats);
+    case 1: return kmc_occupancy_geom_1(plain, lds_floats);
+    case 2: return kmc_occupancy_geom_2(plain, lds_floats);
+    case 3: return kmc_occupancy_geom_3(plain, lds_floats);
     default: return 0;
   }
 }
+
+int kmc_geom_capacity(int geom) { return geom == 0 ? 64 : geom == 1 ? 256 : geom == 2 ? 512 : 1024; }
 
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int* chains, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
@@ -344,8 +382,12 @@ hipError_t launch_corr_reduce(const double* C, const int* stall, int chain0, int
   return hipGetLastError();
 }
 
-hipError_t launch_scan_probe(const float* in, float* out, int n, hipStream_t st) {
-  hipLaunchKernelGGL(scan_probe_kernel, dim3((n + 63) / 64), dim3(64), 0, st, in, out, n);
+hipError_t launch_copy_probe(const void* in, void* out, size_t n4, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, st, (const float4*)in, (float4*)out, n4);
+  return hipGetLastError();
+}
+hipError_t launch_scan_probe(const float* in, float* out, int n, int form, hipStream_t st) {
+  hipLaunchKernelGGL(scan_probe_kernel, dim3(1), dim3(64), 0, st, in, out, n, form);
   return hipGetLastError();
 }
 hipError_t launch_math_probe(const float* in, float* out, int n, int which, hipStream_t st) {

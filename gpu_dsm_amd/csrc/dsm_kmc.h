@@ -1,60 +1,67 @@
-// dsm_kmc.h -- the wave-per-chain KMC step, templated on S (register slots per lane).
-// Included by dsm_kmc_inst.hip (one translation unit per S).
+// dsm_kmc.h -- the KMC jump kernels, templated on the chain geometry.
+// Included by dsm_kmc_inst.hip (one translation unit per geometry).
 //
-// One wave64 owns one chain.  Strand i sits in lane i%64 of register slot i/64, so a
-// chain of up to 64*S strands lives in VGPRs.  A jump (ktaletsk/gpu_dsm
-// common/eq_ensemble_kernel.cu:242-663, common/ensemble_kernel.cu:188-502) is
+// Layout.  A chain's positions (strand i, and the pair/link between strands i and
+// i+1) sit four to a lane: position p = 64*b + 4*l + k is slot k of lane l of the
+// 16-lane row that holds block b of the chain.  Two shapes share that layout:
+//   row form  (CPW = 4): one chain per 16-lane row, four chains per wave, Z <= 64;
+//   wave form (CPW = 1): one chain per wave, block b in row b%4 of register group
+//                        b/4, Z <= 256*G.
+// Every FP32 total (rate sum W, stress samples) uses one canonical tree: serial over a
+// lane's four slots, Hillis-Steele over the 16 lanes of a block, serial carry from
+// block to block.  A chain therefore evolves identically in either shape.
+//
+// A jump (ktaletsk/gpu_dsm common/eq_ensemble_kernel.cu:242-663,
+// common/ensemble_kernel.cu:188-502) is
 //   1. cached pair rates at positions 0..Z-2, the five chain-end rates summed at Z-1,
-//   2. a wave64 inclusive scan (DPP tree) -> total W, dt = 1/W, FP64 clock += dt,
+//   2. the canonical scan -> W, dt = 1/W, FP64 clock += dt,
 //   3. pr = W*u with u from Philox(seed, chain, jump); the pair is the first position
-//      whose inclusive sum exceeds pr (ballot + ffs); the sub-event follows the
-//      reference order CDd, SD+, SD-, CDc (pairs) / CDc tail, SDc head/tail, SDd head/tail,
-//   4. the event is reduced to a small uniform plan (an N shift, or a one-lane gap
-//      opened/closed in every strand array, plus up to two strand and two link
-//      patches) that one shared code path applies, and
-//   5. the cached rates of the pairs next to the change are recomputed.
+//      whose inclusive sum exceeds pr; the sub-event follows the reference order CDd,
+//      SD+, SD-, CDc (pairs) / CDc tail, SDc head, SDc tail, SDd head, SDd tail,
+//   4. the effect is applied to the register-resident chain (an N shift, or a
+//      one-position gap opened/closed in every strand array plus patches), and
+//   5. the cached rates of the (at most three) pairs next to the change are
+//      recomputed in parallel, one pair per lane, from an LDS staging copy.
+//
+// Chain-uniform values (Z, clock, plan of the jump) are per-lane registers that hold
+// the same value on every lane of the chain's segment, so the chain logic is plain
+// per-lane code: rows that take different event branches are masked by exec, and a
+// branch no row takes is skipped.  Cross-lane steps (DPP scans, swizzles,
+// permutes) always run with the full wave active.
 #pragma once
 #include "dsm_device.h"
 #include "dsm_types.h"
 
 namespace dsm {
 
+typedef __attribute__((address_space(3))) float lds_f;
+typedef __attribute__((address_space(3))) int lds_i;
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef int vi4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) vf4 lds_f4;
+typedef __attribute__((address_space(3))) vi4 lds_i4;
+
+// ------------------------------------------------------------------ tables
 // FP32 tables of one ensemble, indexed by the integer strand length N:
 // SP=.75/(N(N+1)), SM=.75/(N(N-1)), PP=(N/(N+1))^.75, PM=(N/(N-1))^.75, EC, ED, ED2
 // (end rates), T3=3/N, CC=c(N-1); then FR[x]=4/x, x=2(f_i+f_j).  Contiguous, so one
-// base pointer (LDS copy or global) and the row length n = nmax+2 address all of them.
+// base pointer and the row length n = nmax+2 address all of them.
+template <bool LDS> struct TabPtr { typedef const __attribute__((address_space(1))) float* type; };
+template <> struct TabPtr<true> { typedef const lds_f* type; };
+
+template <bool LDS>
 struct Tabs {
-  const float* b;
+  typename TabPtr<LDS>::type b;
   int n;
+  __device__ __forceinline__ float at(int i) const { return b[i]; }
   __device__ __forceinline__ float SP(int N) const { return b[N]; }
   __device__ __forceinline__ float SM(int N) const { return b[n + N]; }
   __device__ __forceinline__ float PP(int N) const { return b[2 * n + N]; }
   __device__ __forceinline__ float PM(int N) const { return b[3 * n + N]; }
-  __device__ __forceinline__ float EC(int N) const { return b[4 * n + N]; }
-  __device__ __forceinline__ float ED(int N) const { return b[5 * n + N]; }
-  __device__ __forceinline__ float ED2(int N) const { return b[6 * n + N]; }
   __device__ __forceinline__ float T3(int N) const { return b[7 * n + N]; }
   __device__ __forceinline__ float CC(int N) const { return b[8 * n + N]; }
   __device__ __forceinline__ float FR(int x) const { return b[9 * n + x]; }
 };
-
-// Per-ensemble values used on every jump.  Rarely used ones (kappa, CD lifetime
-// sampler constants) are read from the EnsParams record where they are needed.
-struct EnsLocal {
-  const EnsParams* P;
-  int cd, flow, exp_wait, sample_mode, nchains;
-  uint32_t key0, key1;
-  double delta, t_end;
-  int64_t k0, k_end, stage_off;
-};
-
-__device__ __forceinline__ void load_ens(EnsLocal& E, const EnsParams* __restrict__ ps, int e) {
-  const EnsParams& P = ps[e];
-  E.P = &P;
-  E.cd = P.cd; E.flow = P.flow; E.exp_wait = P.exp_wait; E.sample_mode = P.sample_mode;
-  E.nchains = P.nchains; E.key0 = P.key0; E.key1 = P.key1; E.delta = P.delta; E.t_end = P.t_end;
-  E.k0 = P.k0; E.k_end = P.k_end; E.stage_off = P.stage_off;
-}
 
 // CD lifetime samplers returning the rate 1/tau (common/gpu_random.cu:143-148)
 __device__ __forceinline__ float rate_ft(const EnsParams* P, float u) {
@@ -66,9 +73,9 @@ __device__ __forceinline__ float rate_fdt(const EnsParams* P, float u) {
 
 // Sliding-dynamics shift rates of the pair (i, j=i+1) and the pair's total with its
 // CD terms (common/eq_ensemble_kernel.cu:97-126).
-__device__ __forceinline__ void pair_rate(const Tabs& T, int cd, int Ni, int Nj, float q2i, float q2j,
-                                          float we, bool endi, bool endj, float& wp, float& wm,
-                                          float& R) {
+template <bool LDS>
+__device__ __forceinline__ void pair_rate(const Tabs<LDS>& T, int cd, int Ni, int Nj, float q2i, float q2j,
+                                          float we, bool endi, bool endj, float& wp, float& wm, float& R) {
   wp = 0.0f;
   wm = 0.0f;
   if (Nj > 1) {
@@ -92,119 +99,6 @@ __device__ __forceinline__ float q2_of(float x, float y, float z) {
   return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
 }
 
-template <int S>
-struct Chain {
-  float qx[S], qy[S], qz[S], q2[S], w[S], wp[S], wm[S], rr[S];
-  int n[S];
-  int Z;
-};
-
-// record at the (uniform) position pos
-template <int S, typename T>
-__device__ __forceinline__ T rd(const T (&a)[S], int pos) {
-  const int s = pos >> 6;
-  T v = a[0];
-#pragma unroll
-  for (int k = 1; k < S; ++k) v = (s == k) ? a[k] : v;
-  return rdlane(v, pos & 63);
-}
-
-// new[i] = (i > at && i <= Z) ? old[i-1] : old[i]
-template <int S, typename T>
-__device__ __forceinline__ void gap_open_arr(T (&a)[S], int at, int Z) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int s = S - 1; s >= 0; --s) {
-    const T fill = (s > 0) ? rdlane(a[s > 0 ? s - 1 : 0], 63) : (T)0;
-    const T pv = from_prev_lane(a[s], fill);
-    const int i = s * 64 + lane;
-    a[s] = (i > at && i <= Z) ? pv : a[s];
-  }
-}
-// new[i] = (i >= at && i < Z-1) ? old[i+1] : old[i]
-template <int S, typename T>
-__device__ __forceinline__ void gap_close_arr(T (&a)[S], int at, int Z) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const T fill = (s + 1 < S) ? rdlane(a[s + 1 < S ? s + 1 : s], 0) : (T)0;
-    const T nv = from_next_lane(a[s], fill);
-    const int i = s * 64 + lane;
-    a[s] = (i >= at && i < Z - 1) ? nv : a[s];
-  }
-}
-
-// recompute cached rates of pairs lo..hi (clamped to 0..Z-2)
-template <int S>
-__device__ __forceinline__ void refresh(Chain<S>& C, const Tabs& T, int cd, int lo, int hi) {
-  if (lo < 0) lo = 0;
-  if (hi > C.Z - 2) hi = C.Z - 2;
-  if (lo > hi) return;
-  const int lane = lane_id();
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (s * 64 > hi || s * 64 + 63 < lo) continue;
-    const int nfill = (s + 1 < S) ? rdlane(C.n[s + 1 < S ? s + 1 : s], 0) : 0;
-    const float qfill = (s + 1 < S) ? rdlane(C.q2[s + 1 < S ? s + 1 : s], 0) : 0.0f;
-    const int nj = from_next_lane(C.n[s], nfill);
-    const float q2j = from_next_lane(C.q2[s], qfill);
-    const int i = s * 64 + lane;
-    if (i >= lo && i <= hi) {
-      float wp, wm, R;
-      pair_rate(T, cd, C.n[s], nj, C.q2[s], q2j, C.w[s], i == 0, i + 1 == C.Z - 1, wp, wm, R);
-      C.wp[s] = wp; C.wm[s] = wm; C.rr[s] = R;
-    }
-  }
-}
-
-// inclusive tree-sum value at position Z-1 of per-position values v (zero beyond Z)
-template <int S>
-__device__ __forceinline__ float tree_total(const float (&v)[S], int Z) {
-  const int last = Z - 1, ls = last >> 6;
-  float carry = 0.0f, res = 0.0f;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (s <= ls) {
-      const float sc = wave_scan(v[s]);
-      const float at = rdlane(sc, last & 63), top = rdlane(sc, 63);
-      if (s == ls) res = (s == 0) ? at : carry + at;
-      carry = (s == 0) ? top : carry + top;
-    }
-  }
-  return res;
-}
-
-// chain-end rates (common/eq_ensemble_kernel.cu:340-392): cdcz sdc1 sdcz sdd1 sddz
-template <int S>
-__device__ __forceinline__ float end_rates(const Chain<S>& C, const Tabs& T, float e1, float er[5]) {
-  const int Z = C.Z;
-  float cdcz = 0.0f, sdc1 = 0.0f, sdcz = 0.0f, sdd1 = 0.0f, sddz = 0.0f;
-  if (Z == 1) {
-    cdcz = T.CC(rd(C.n, 0));
-    sdc1 = e1;
-    sdcz = e1;
-  } else {
-    const int N0 = rd(C.n, 0), N1 = rd(C.n, 1), Nz = rd(C.n, Z - 1), Nz2 = rd(C.n, Z - 2);
-    if (N0 == 1) sdd1 = (Z == 2) ? T.ED2(N1) : T.ED(N1);
-    else sdc1 = T.EC(N0);
-    if (Nz == 1) sddz = (Z == 2) ? T.ED2(Nz2) : T.ED(Nz2);
-    else sdcz = T.EC(Nz);
-    cdcz = T.CC(Nz);
-  }
-  er[0] = cdcz; er[1] = sdc1; er[2] = sdcz; er[3] = sdd1; er[4] = sddz;
-  return (((cdcz + sdc1) + sdcz) + sdd1) + sddz;
-}
-
-__device__ __forceinline__ int pick_sub(const float* s, int n, float& r) {
-  float rr = r;
-  for (int m = 0; m < n; ++m) {
-    if (s[m] > 0.0f && rr < s[m]) { r = rr; return m; }
-    rr = rr - s[m];
-  }
-  r = rr;
-  return -1;
-}
-
 __device__ __forceinline__ int split_newn(float r, int N, float wc) {
   int newn = (int)floorf(0.5f + (r * (float)(N - 2)) / wc) + 1;
   newn = newn < 1 ? 1 : newn;
@@ -212,15 +106,297 @@ __device__ __forceinline__ int split_newn(float r, int N, float wc) {
   return newn;
 }
 
-// Philox outputs for 64 consecutive jumps of the chain, one jump per lane.
-struct RandCache {
-  float sel, wait, tau, g0, g1, g2;
-  uint64_t base;
-  bool valid;
+// ------------------------------------------------------------------ geometry
+template <int CPW, int G>
+struct Geo {
+  static_assert((CPW == 4 && G == 1) || (CPW == 1 && G >= 1 && G <= 4), "unsupported chain geometry");
+  static constexpr int R = 4 * G;                        // registers per strand array
+  static constexpr int SEG = CPW == 4 ? 16 : 64;         // lanes per chain
+  static constexpr int CAP = CPW == 4 ? 64 : 256 * G;    // positions per chain
+  static constexpr int STG = CPW == 4 ? 256 : 256 * G;   // floats per staging array
 };
 
-__device__ __forceinline__ void refill_rand(RandCache& R, uint64_t ev, uint32_t cid, const EnsLocal& E) {
-  const uint64_t e = ev + (uint64_t)lane_id();
+// LDS staging arrays of a wave (each STG floats; chain positions at srow + p)
+enum StageArray : int { kStN = 0, kStQ2, kStW, kStWP, kStWM, kStRR, kStG, kStArrays };
+
+template <int CPW>
+struct LaneC {
+  int lane;  // 0..63
+  int sl;    // lane within the chain's segment
+  int sb;    // first lane of the segment
+  int pos0;  // position of slot 0 of register group 0
+  int srow;  // staging offset of the chain's position 0
+  __device__ __forceinline__ LaneC() {
+    lane = lane_id();
+    sl = CPW == 4 ? (lane & 15) : lane;
+    sb = CPW == 4 ? (lane & 48) : 0;
+    pos0 = 4 * sl;
+    srow = CPW == 4 ? 64 * (lane >> 4) : 0;
+  }
+  __device__ __forceinline__ int pos(int j) const { return 256 * (j >> 2) + pos0 + (j & 3); }
+};
+
+// ------------------------------------------------------------------ cross-lane steps
+template <int CTRL>
+__device__ __forceinline__ int dppi(int v, int old) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v, float old) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118, kRowShl1 = 0x101;
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
+
+// value of the previous / next lane of the chain's segment (0 outside the segment)
+template <int CPW, typename T>
+__device__ __forceinline__ T seg_prev(T v);
+template <int CPW, typename T>
+__device__ __forceinline__ T seg_next(T v);
+template <> __device__ __forceinline__ float seg_prev<4, float>(float v) { return dppf<kRowShr1>(v, 0.0f); }
+template <> __device__ __forceinline__ float seg_prev<1, float>(float v) { return dppf<kWaveShr1>(v, 0.0f); }
+template <> __device__ __forceinline__ int seg_prev<4, int>(int v) { return dppi<kRowShr1>(v, 0); }
+template <> __device__ __forceinline__ int seg_prev<1, int>(int v) { return dppi<kWaveShr1>(v, 0); }
+template <> __device__ __forceinline__ float seg_next<4, float>(float v) { return dppf<kRowShl1>(v, 0.0f); }
+template <> __device__ __forceinline__ float seg_next<1, float>(float v) { return dppf<kWaveShl1>(v, 0.0f); }
+template <> __device__ __forceinline__ int seg_next<4, int>(int v) { return dppi<kRowShl1>(v, 0); }
+template <> __device__ __forceinline__ int seg_next<1, int>(int v) { return dppi<kWaveShl1>(v, 0); }
+
+__device__ __forceinline__ float rdl(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int rdl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// value of v on segment lane `src` (per lane; 0 <= src < SEG)
+template <int CPW>
+__device__ __forceinline__ float seg_get(float v, int src, const LaneC<CPW>& L) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute((L.sb + src) << 2, __float_as_int(v)));
+}
+template <int CPW>
+__device__ __forceinline__ int seg_get(int v, int src, const LaneC<CPW>& L) {
+  return __builtin_amdgcn_ds_bpermute((L.sb + src) << 2, v);
+}
+
+// lane 15 of each 16-lane row, broadcast over the row (bitmask swizzle and=0x10 or=0x0f)
+__device__ __forceinline__ float row_last(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F0));
+}
+__device__ __forceinline__ int row_last(int v) { return __builtin_amdgcn_ds_swizzle(v, 0x1F0); }
+
+// canonical block scan of one register group: inclusive values I[0..3]
+__device__ __forceinline__ void block_scan4(float v0, float v1, float v2, float v3, float I[4]) {
+  const float s0 = v0, s1 = s0 + v1, s2 = s1 + v2, s3 = s2 + v3;
+  float H = s3;
+  H = H + dppf<kRowShr1>(H, 0.0f);
+  H = H + dppf<kRowShr2>(H, 0.0f);
+  H = H + dppf<kRowShr4>(H, 0.0f);
+  H = H + dppf<kRowShr8>(H, 0.0f);
+  const float X = dppf<kRowShr1>(H, 0.0f);
+  I[0] = X + s0;
+  I[1] = X + s1;
+  I[2] = X + s2;
+  I[3] = X + s3;
+}
+
+// Canonical inclusive sums G of v over the chain (v is zero at positions >= Z) and
+// the chain total (G at the last position of the last occupied block).
+template <int CPW, int G>
+__device__ __forceinline__ float chain_scan(const float (&v)[4 * G], float (&out)[4 * G], int Z,
+                                            const LaneC<CPW>& L) {
+  if constexpr (CPW == 4) {
+    block_scan4(v[0], v[1], v[2], v[3], out);
+    return row_last(out[3]);
+  } else {
+    const int nb = ((__builtin_amdgcn_readfirstlane(Z) - 1) >> 6) + 1;
+    const int row = L.lane >> 4;
+    float carry = 0.0f, total = 0.0f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      float I[4];
+      block_scan4(v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3], I);
+      float cr = 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = 4 * g + r;
+        const float Bt = rdl(I[3], 16 * r + 15);
+        cr = (row == r) ? carry : cr;
+        carry = (b == 0) ? Bt : carry + Bt;
+        if (b == nb - 1) total = carry;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out[4 * g + k] = (g == 0 && row == 0) ? I[k] : cr + I[k];
+    }
+    return total;
+  }
+}
+
+template <int CPW, int G>
+__device__ __forceinline__ float chain_total(const float (&v)[4 * G], int Z, const LaneC<CPW>& L) {
+  float tmp[4 * G];
+  return chain_scan<CPW, G>(v, tmp, Z, L);
+}
+
+// First position p < Z with G_p > pr, replicated over the segment; CAP when none.
+template <int CPW, int G>
+__device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr, int Z, const LaneC<CPW>& L) {
+  constexpr int CAP = Geo<CPW, G>::CAP;
+  if constexpr (CPW == 4) {
+    int cand = CAP;
+#pragma unroll
+    for (int k = 3; k >= 0; --k) cand = (Gv[k] > pr && L.pos0 + k < Z) ? L.pos0 + k : cand;
+    cand = min(cand, dppi<kRowShr1>(cand, CAP));
+    cand = min(cand, dppi<kRowShr2>(cand, CAP));
+    cand = min(cand, dppi<kRowShr4>(cand, CAP));
+    cand = min(cand, dppi<kRowShr8>(cand, CAP));
+    return row_last(cand);
+  } else {
+    int res = CAP;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      int kf = 4;
+#pragma unroll
+      for (int k = 3; k >= 0; --k) kf = (Gv[4 * g + k] > pr && 256 * g + L.pos0 + k < Z) ? k : kf;
+      const unsigned long long m = __ballot(kf < 4);
+      if (res == CAP && m) {
+        const int ln = __ffsll((long long)m) - 1;
+        res = 256 * g + 4 * ln + rdl(kf, ln);
+      }
+    }
+    return res;
+  }
+}
+
+// value of register array a at position p (p replicated over the segment, 0 <= p < CAP).
+// Every register is permuted and the slot is selected afterwards, so the array is
+// never indexed at run time (which would force it out of registers).
+template <int CPW, int G, typename T>
+__device__ __forceinline__ T value_at(const T (&a)[4 * G], int p, const LaneC<CPW>& L) {
+  const int j = 4 * (p >> 8) + (p & 3);
+  const int src = (p >> 2) & (Geo<CPW, G>::SEG - 1);
+  T v = seg_get<CPW>(a[0], src, L);
+#pragma unroll
+  for (int q = 1; q < 4 * G; ++q) {
+    const T x = seg_get<CPW>(a[q], src, L);
+    v = (j == q) ? x : v;
+  }
+  return v;
+}
+
+// new[p] = (at < p <= Z) ? old[p-1] : old[p]   (at, Z replicated; at >= CAP: no-op)
+template <int CPW, int G, typename T>
+__device__ __forceinline__ void gap_open(T (&a)[4 * G], int at, int Z, const LaneC<CPW>& L) {
+  T prev3[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    T pv = seg_prev<CPW, T>(a[4 * g + 3]);
+    if (CPW == 1 && g > 0) pv = (L.lane == 0) ? rdl(a[4 * g - 1], 63) : pv;
+    prev3[g] = pv;
+  }
+#pragma unroll
+  for (int j = 4 * G - 1; j >= 0; --j) {
+    const int p = L.pos(j);
+    const T src = (j & 3) ? a[j > 0 ? j - 1 : 0] : prev3[j >> 2];
+    a[j] = (p > at && p <= Z) ? src : a[j];
+  }
+}
+// new[p] = (at <= p < Z-1) ? old[p+1] : old[p]
+template <int CPW, int G, typename T>
+__device__ __forceinline__ void gap_close(T (&a)[4 * G], int at, int Z, const LaneC<CPW>& L) {
+  T next0[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    T nv = seg_next<CPW, T>(a[4 * g]);
+    if (CPW == 1 && g + 1 < G) nv = (L.lane == 63) ? rdl(a[4 * g + 4 < 4 * G ? 4 * g + 4 : 0], 0) : nv;
+    next0[g] = nv;
+  }
+#pragma unroll
+  for (int j = 0; j < 4 * G; ++j) {
+    const int p = L.pos(j);
+    const T src = ((j & 3) != 3) ? a[j + 1 < 4 * G ? j + 1 : j] : next0[j >> 2];
+    a[j] = (p >= at && p < Z - 1) ? src : a[j];
+  }
+}
+
+// ------------------------------------------------------------------ per-wave LDS staging
+template <int CPW, int G>
+struct Stage {
+  lds_f* s;  // wave base
+  int srow;
+  __device__ __forceinline__ lds_f* arr(int a) const { return s + a * Geo<CPW, G>::STG + srow; }
+  __device__ __forceinline__ float getf(int a, int p) const { return arr(a)[p]; }
+  __device__ __forceinline__ int geti(int a, int p) const { return ((lds_i*)arr(a))[p]; }
+  __device__ __forceinline__ void putf(int a, int p, float v) const { arr(a)[p] = v; }
+  // this lane's four slots of register group g
+  __device__ __forceinline__ void put4(int a, int g, int pos0, float v0, float v1, float v2, float v3) const {
+    const vf4 v = {v0, v1, v2, v3};
+    *(lds_f4*)(arr(a) + 256 * g + pos0) = v;
+  }
+  __device__ __forceinline__ void put4i(int a, int g, int pos0, int v0, int v1, int v2, int v3) const {
+    const vi4 v = {v0, v1, v2, v3};
+    *(lds_i4*)((lds_i*)arr(a) + 256 * g + pos0) = v;
+  }
+  __device__ __forceinline__ vf4 get4(int a, int g, int pos0) const {
+    return *(lds_f4*)(arr(a) + 256 * g + pos0);
+  }
+};
+
+// orders this wave's LDS writes before its later reads (cross-lane exchange)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------------ chain state
+template <int G>
+struct Strands {
+  float qx[4 * G], qy[4 * G], qz[4 * G], q2[4 * G], w[4 * G];
+  int n[4 * G];
+};
+
+// Chain-uniform values, replicated over the chain's lanes.
+struct Hdr {
+  int c;          // chain index, -1: empty segment
+  int inert;      // empty, stalled or over capacity at load, or overflowed since: no samples, no jumps
+  int Z, nk, stall, local;
+  uint32_t cid;
+  int64_t next_sample;
+  double t, t_next;
+  uint64_t ev, zsum;
+  int left;       // jumps left before the chain's ev_stop (saturated)
+  float dt_prev, e1;
+  int ovf, done, dlo, dhi;
+};
+
+// Per-wave ensemble values (every chain of a wave belongs to one ensemble).
+struct EnsLocal {
+  const EnsParams* P;
+  int cd, flow, exp_wait, sample_mode, nchains, nmax;
+  uint32_t key0, key1;
+  double delta, t_end;
+  int64_t k0, k_end, stage_off;
+  float beta;
+};
+
+__device__ __forceinline__ void load_ens(EnsLocal& E, const EnsParams* __restrict__ ps, int e) {
+  const EnsParams& P = ps[e];
+  E.P = &P;
+  E.cd = P.cd; E.flow = P.flow; E.exp_wait = P.exp_wait; E.sample_mode = P.sample_mode;
+  E.nchains = P.nchains; E.nmax = P.nmax; E.key0 = P.key0; E.key1 = P.key1; E.delta = P.delta;
+  E.t_end = P.t_end; E.k0 = P.k0; E.k_end = P.k_end; E.stage_off = P.stage_off; E.beta = P.beta;
+}
+
+__device__ __forceinline__ double next_time(const EnsLocal& E, int64_t next_sample) {
+  return next_sample < E.k_end ? (double)next_sample * E.delta : __longlong_as_double(0x7ff0000000000000ll);
+}
+
+// Philox outputs for SEG consecutive jumps of the chain, jump base+sl on segment lane sl.
+struct RandCache {
+  float sel, wait, tau, g0, g1, g2;
+};
+
+__device__ __forceinline__ void refill_rand(RandCache& R, uint64_t ev, int sl, uint32_t cid, const EnsLocal& E) {
+  const uint64_t e = ev + (uint64_t)sl;
   const uint32_t lo = (uint32_t)e, hi = (uint32_t)(e >> 32);
   const uint4 x = philox4x32_10(make_uint4(lo, hi, cid, kPurposeStep), E.key0, E.key1);
   R.sel = u01(x.x); R.wait = u01(x.y); R.tau = u01(x.z);
@@ -228,59 +404,146 @@ __device__ __forceinline__ void refill_rand(RandCache& R, uint64_t ev, uint32_t 
   float g3;
   boxmuller(g.x, g.y, R.g0, R.g1);
   boxmuller(g.z, g.w, R.g2, g3);
-  R.base = ev;
-  R.valid = true;
+}
+
+// ------------------------------------------------------------------ building blocks
+// Stage the strand arrays used by the rate refresh and end rates.
+template <int CPW, int G>
+__device__ __forceinline__ void stage_n(const Stage<CPW, G>& S, const Strands<G>& C, const LaneC<CPW>& L) {
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    S.put4i(kStN, g, L.pos0, C.n[4 * g], C.n[4 * g + 1], C.n[4 * g + 2], C.n[4 * g + 3]);
+}
+template <int CPW, int G>
+__device__ __forceinline__ void stage_qw(const Stage<CPW, G>& S, const Strands<G>& C, const LaneC<CPW>& L) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    S.put4(kStQ2, g, L.pos0, C.q2[4 * g], C.q2[4 * g + 1], C.q2[4 * g + 2], C.q2[4 * g + 3]);
+    S.put4(kStW, g, L.pos0, C.w[4 * g], C.w[4 * g + 1], C.w[4 * g + 2], C.w[4 * g + 3]);
+  }
+}
+
+// Recompute every cached pair rate of the chain (after a load or a deformation):
+// each lane evaluates the pairs of its own slots.
+template <int CPW, int G, bool LT>
+__device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strands<G>& C, const Tabs<LT>& T, int cd,
+                                            int Z, const LaneC<CPW>& L) {
+  int nn[G];
+  float qn[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    int a = seg_next<CPW, int>(C.n[4 * g]);
+    float b = seg_next<CPW, float>(C.q2[4 * g]);
+    if (CPW == 1 && g + 1 < G) {
+      const int gn = g + 1 < G ? g + 1 : g;
+      a = (L.lane == 63) ? rdl(C.n[4 * gn], 0) : a;
+      b = (L.lane == 63) ? rdl(C.q2[4 * gn], 0) : b;
+    }
+    nn[g] = a;
+    qn[g] = b;
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    float wp[4], wm[4], rr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int j = 4 * g + k;
+      const int p = L.pos(j);
+      const int Nj = (k < 3) ? C.n[j + (k < 3 ? 1 : 0)] : nn[g];
+      const float q2j = (k < 3) ? C.q2[j + (k < 3 ? 1 : 0)] : qn[g];
+      wp[k] = 0.0f; wm[k] = 0.0f; rr[k] = 0.0f;
+      if (p <= Z - 2) pair_rate<LT>(T, cd, C.n[j], Nj, C.q2[j], q2j, C.w[j], p == 0, p + 1 == Z - 1, wp[k], wm[k], rr[k]);
+    }
+    S.put4(kStWP, g, L.pos0, wp[0], wp[1], wp[2], wp[3]);
+    S.put4(kStWM, g, L.pos0, wm[0], wm[1], wm[2], wm[3]);
+    S.put4(kStRR, g, L.pos0, rr[0], rr[1], rr[2], rr[3]);
+  }
+}
+
+// Recompute the cached rates of pairs lo..hi (at most three, clamped to 0..Z-2) in
+// lanes 0..2 of the segment; lane 3 zeroes the rate cache at zero_at (>= 0).
+template <int CPW, int G, bool LT>
+__device__ __forceinline__ void refresh_pairs(const Stage<CPW, G>& S, const Tabs<LT>& T, int cd, int Z, int lo, int hi,
+                                              int zero_at, const LaneC<CPW>& L) {
+  const int q = lo + L.sl;
+  if (L.sl < 3 && q <= hi && q >= 0 && q <= Z - 2) {
+    const int Ni = S.geti(kStN, q), Nj = S.geti(kStN, q + 1);
+    const float q2i = S.getf(kStQ2, q), q2j = S.getf(kStQ2, q + 1), we = S.getf(kStW, q);
+    float wp, wm, R;
+    pair_rate<LT>(T, cd, Ni, Nj, q2i, q2j, we, q == 0, q + 1 == Z - 1, wp, wm, R);
+    S.putf(kStWP, q, wp);
+    S.putf(kStWM, q, wm);
+    S.putf(kStRR, q, R);
+  }
+  if (L.sl == 3 && zero_at >= 0) S.putf(kStRR, zero_at, 0.0f);
+}
+
+// Gap op on the LDS-resident rate caches (wp, wm, rr).
+template <int CPW, int G>
+__device__ __forceinline__ void gap_lds(const Stage<CPW, G>& S, int dir, int at, int Z, const LaneC<CPW>& L) {
+#pragma unroll
+  for (int a = kStWP; a <= kStRR; ++a) {
+    float v[4 * G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const vf4 x = S.get4(a, g, L.pos0);
+      v[4 * g] = x.x; v[4 * g + 1] = x.y; v[4 * g + 2] = x.z; v[4 * g + 3] = x.w;
+    }
+    if (dir > 0) gap_open<CPW, G>(v, at, Z, L);
+    else gap_close<CPW, G>(v, at, Z, L);
+#pragma unroll
+    for (int g = 0; g < G; ++g) S.put4(a, g, L.pos0, v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3]);
+  }
 }
 
 // Flow: explicit Euler affine step Q <- Q + dt*kappa*Q (common/ensemble_kernel.cu:105-110)
-template <int S>
-__device__ __forceinline__ void deform(Chain<S>& C, const EnsParams* P, float dt) {
+template <int CPW, int G>
+__device__ __forceinline__ void deform(Strands<G>& C, const EnsParams* P, float dt, int Z, const LaneC<CPW>& L) {
   float dk[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) dk[i] = dt * P->kappa[i];
-  const int lane = lane_id();
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (s * 64 + lane < C.Z) {
-      const float x = C.qx[s], y = C.qy[s], z = C.qz[s];
-      C.qx[s] = __builtin_fmaf(dk[2], z, __builtin_fmaf(dk[1], y, __builtin_fmaf(dk[0], x, x)));
-      C.qy[s] = __builtin_fmaf(dk[5], z, __builtin_fmaf(dk[4], y, __builtin_fmaf(dk[3], x, y)));
-      C.qz[s] = __builtin_fmaf(dk[8], z, __builtin_fmaf(dk[7], y, __builtin_fmaf(dk[6], x, z)));
-      C.q2[s] = q2_of(C.qx[s], C.qy[s], C.qz[s]);
+  for (int j = 0; j < 4 * G; ++j) {
+    if (L.pos(j) < Z) {
+      const float x = C.qx[j], y = C.qy[j], z = C.qz[j];
+      C.qx[j] = __builtin_fmaf(dk[2], z, __builtin_fmaf(dk[1], y, __builtin_fmaf(dk[0], x, x)));
+      C.qy[j] = __builtin_fmaf(dk[5], z, __builtin_fmaf(dk[4], y, __builtin_fmaf(dk[3], x, y)));
+      C.qz[j] = __builtin_fmaf(dk[8], z, __builtin_fmaf(dk[7], y, __builtin_fmaf(dk[6], x, z)));
+      C.q2[j] = q2_of(C.qx[j], C.qy[j], C.qz[j]);
     }
   }
 }
 
 // EQ sample (sxy, syz, sxz) = -sum 3 Qa Qb / N  (common/eq_ensemble_kernel.cu:281-296)
-template <int S>
-__device__ __forceinline__ void eq_sample(const Chain<S>& C, const Tabs& T, float out[3]) {
-  float a[S], b[S], d[S];
-  const int lane = lane_id();
+template <int CPW, int G, bool LT>
+__device__ __forceinline__ void eq_sample(const Strands<G>& C, const Tabs<LT>& T, int Z, const LaneC<CPW>& L,
+                                          float out[3]) {
+  float a[4 * G], b[4 * G], d[4 * G];
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const bool on = s * 64 + lane < C.Z;
-    const float t3 = on ? T.T3(C.n[s]) : 0.0f;
-    a[s] = on ? (C.qx[s] * C.qy[s]) * t3 : 0.0f;
-    b[s] = on ? (C.qy[s] * C.qz[s]) * t3 : 0.0f;
-    d[s] = on ? (C.qx[s] * C.qz[s]) * t3 : 0.0f;
+  for (int j = 0; j < 4 * G; ++j) {
+    const bool on = L.pos(j) < Z;
+    const float t3 = on ? T.T3(C.n[j]) : 0.0f;
+    a[j] = on ? (C.qx[j] * C.qy[j]) * t3 : 0.0f;
+    b[j] = on ? (C.qy[j] * C.qz[j]) * t3 : 0.0f;
+    d[j] = on ? (C.qx[j] * C.qz[j]) * t3 : 0.0f;
   }
-  out[0] = -tree_total<S>(a, C.Z);
-  out[1] = -tree_total<S>(b, C.Z);
-  out[2] = -tree_total<S>(d, C.Z);
+  out[0] = -chain_total<CPW, G>(a, Z, L);
+  out[1] = -chain_total<CPW, G>(b, Z, L);
+  out[2] = -chain_total<CPW, G>(d, Z, L);
 }
 
 // flow sample: xx yy zz xy yz zx Lpp Z (common/ensemble_kernel.cu:504-538)
-template <int S>
-__device__ __forceinline__ void flow_sample(const Chain<S>& C, const Tabs& T, float out[8]) {
-  const int lane = lane_id();
+template <int CPW, int G, bool LT>
+__device__ __forceinline__ void flow_sample(const Strands<G>& C, const Tabs<LT>& T, int Z, const LaneC<CPW>& L,
+                                            float out[8]) {
 #pragma unroll
   for (int comp = 0; comp < 7; ++comp) {
-    float v[S];
+    float v[4 * G];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const bool on = s * 64 + lane < C.Z;
-      const float x = C.qx[s], y = C.qy[s], z = C.qz[s];
-      const float t3 = on ? T.T3(C.n[s]) : 0.0f;
+    for (int j = 0; j < 4 * G; ++j) {
+      const bool on = L.pos(j) < Z;
+      const float x = C.qx[j], y = C.qy[j], z = C.qz[j];
+      const float t3 = on ? T.T3(C.n[j]) : 0.0f;
       float val;
       switch (comp) {
         case 0: val = (x * x) * t3; break;
@@ -291,410 +554,468 @@ __device__ __forceinline__ void flow_sample(const Chain<S>& C, const Tabs& T, fl
         case 5: val = (x * z) * t3; break;
         default: val = sqrtf(q2_of(x, y, z)); break;
       }
-      v[s] = on ? val : 0.0f;
+      v[j] = on ? val : 0.0f;
     }
-    const float tot = tree_total<S>(v, C.Z);
+    const float tot = chain_total<CPW, G>(v, Z, L);
     out[comp] = comp < 6 ? -tot : tot;
   }
-  out[7] = (float)C.Z;
+  out[7] = (float)Z;
 }
 
-// Record every pending sample: G(t) sample k while t > k*delta (the state after the
-// first jump past k*delta); flow output k while t >= k*delta.
-template <int S>
-__device__ __forceinline__ void record_samples(const Chain<S>& C, const Tabs& T, const EnsLocal& E,
-                                               double t, int64_t& next_sample, int local, float* stage) {
+// Record every pending sample of every chain of the wave: G(t) sample k while
+// t > k*delta (the state after the first jump past k*delta); flow output k while
+// t >= k*delta.
+template <int CPW, int G, bool LT>
+__device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<LT>& T, const EnsLocal& E, Hdr& H,
+                                               float* stage, const LaneC<CPW>& L) {
   if (E.sample_mode == kSampleEq) {
-    while (next_sample < E.k_end && t > (double)next_sample * E.delta) {
+    for (;;) {
+      const bool pend = !H.inert && H.t > H.t_next;
+      if (!__ballot(pend)) break;
       float sv[3];
-      eq_sample<S>(C, T, sv);
-      if (lane_id() == 0 && next_sample >= E.k0) {
-        float* o = stage + E.stage_off + ((next_sample - E.k0) * E.nchains + local) * 3;
-        o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+      eq_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+      if (pend) {
+        if (L.sl == 0 && H.next_sample >= E.k0) {
+          float* o = stage + E.stage_off + ((H.next_sample - E.k0) * E.nchains + H.local) * 3;
+          o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+        }
+        ++H.next_sample;
+        H.t_next = next_time(E, H.next_sample);
       }
-      ++next_sample;
     }
   } else if (E.sample_mode == kSampleFlow) {
-    while (next_sample < E.k_end && t >= (double)next_sample * E.delta) {
+    for (;;) {
+      const bool pend = !H.inert && H.t >= H.t_next;
+      if (!__ballot(pend)) break;
       float sv[8];
-      flow_sample<S>(C, T, sv);
-      if (lane_id() == 0 && next_sample >= E.k0) {
-        float* o = stage + E.stage_off + ((next_sample - E.k0) * E.nchains + local) * 8;
+      flow_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+      if (pend) {
+        if (L.sl == 0 && H.next_sample >= E.k0) {
+          float* o = stage + E.stage_off + ((H.next_sample - E.k0) * E.nchains + H.local) * 8;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) o[q] = sv[q];
+          for (int q = 0; q < 8; ++q) o[q] = sv[q];
+        }
+        ++H.next_sample;
+        H.t_next = next_time(E, H.next_sample);
       }
-      ++next_sample;
     }
   }
 }
 
-// A jump's effect as uniform values (see the header comment).
-struct Plan {
-  int kind;
-  int gap;           // 0 none, +1 open at `at`, -1 close at `at`
-  int at;
-  int znew;
-  int dn_idx, dn;    // N[dn_idx] += dn, N[dn_idx+1] -= dn
-  int p0, p0N;       // strand patch 0 (p0 < 0: none)
-  float p0x, p0y, p0z;
-  int p1, p1N;       // strand patch 1
-  float p1x, p1y, p1z;
-  int w0i, w1i;      // link patches
-  float w0, w1;
-  int lo, hi;        // pairs to recompute
-  int dlo, dhi;      // strand records to write back (plain form)
-};
-
-// Steps 1-3: pick the jump and describe its effect.  Advances t, ev, dt_prev.
-template <int S>
-__device__ __forceinline__ void kmc_pick(const Chain<S>& C, const Tabs& T, const EnsLocal& E, float e1,
-                                         uint32_t cid, RandCache& RC, double& t, uint64_t& ev, float& dt_prev,
-                                         int& stall, Plan& L) {
-  const int lane = lane_id();
-  const int Z = C.Z;
-  L.kind = kEvNone; L.gap = 0; L.at = 0; L.znew = Z; L.dn = 0; L.dn_idx = 0;
-  L.p0 = -1; L.p1 = -1; L.w0i = -1; L.w1i = -1; L.lo = 1; L.hi = 0; L.dlo = 1 << 30; L.dhi = -1;
-  L.p0N = L.p1N = 0; L.p0x = L.p0y = L.p0z = L.p1x = L.p1y = L.p1z = L.w0 = L.w1 = 0.0f;
-  float er[5];
-  const float Eend = end_rates<S>(C, T, e1, er);
-  float incl[S];
+// ------------------------------------------------------------------ one jump of every active chain
+template <int CPW, int G, bool LT>
+__device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stage<CPW, G>& S, const Tabs<LT>& T,
+                                     const EnsLocal& E, RandCache& RC, int& k_rng, const LaneC<CPW>& L) {
+  constexpr int SEG = Geo<CPW, G>::SEG;
+  constexpr int CAP = Geo<CPW, G>::CAP;
+  const int Z = H.Z;
+  if (k_rng == SEG) {
+    refill_rand(RC, H.ev, L.sl, H.cid, E);
+    k_rng = 0;
+  }
+  // 1. rates at positions: cached pair rates, the chain-end rates at Z-1
+  float v[4 * G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const vf4 x = S.get4(kStRR, g, L.pos0);
+    v[4 * g] = x.x; v[4 * g + 1] = x.y; v[4 * g + 2] = x.z; v[4 * g + 3] = x.w;
+  }
+  // chain-end rates (common/eq_ensemble_kernel.cu:340-392): cdcz sdc1 sdcz sdd1 sddz
+  const int N0 = S.geti(kStN, 0), N1 = S.geti(kStN, 1);
+  const int Nz = S.geti(kStN, Z - 1), Nz2 = S.geti(kStN, Z >= 2 ? Z - 2 : 0);
+  float cdcz, sdc1, sdcz, sdd1, sddz;
   {
-    float carry = 0.0f;
-    const int ls = (Z - 1) >> 6;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      incl[s] = 0.0f;
-      if (s <= ls) {
-        const int i = s * 64 + lane;
-        const float pos = (i < Z - 1) ? C.rr[s] : (i == Z - 1 ? Eend : 0.0f);
-        const float sc = wave_scan(pos);
-        incl[s] = (s == 0) ? sc : carry + sc;
-        const float top = rdlane(sc, 63);
-        carry = (s == 0) ? top : carry + top;
-      }
+    const bool z2 = (Z == 2);
+    const int ho = (N0 == 1) ? (z2 ? 6 : 5) * T.n + N1 : 4 * T.n + N0;
+    const int to = (Nz == 1) ? (z2 ? 6 : 5) * T.n + Nz2 : 4 * T.n + Nz;
+    const float h = T.at(ho), tt = T.at(to);
+    cdcz = T.CC(Nz);
+    if (Z == 1) {
+      sdc1 = H.e1; sdcz = H.e1; sdd1 = 0.0f; sddz = 0.0f;
+    } else {
+      sdd1 = (N0 == 1) ? h : 0.0f;
+      sdc1 = (N0 == 1) ? 0.0f : h;
+      sddz = (Nz == 1) ? tt : 0.0f;
+      sdcz = (Nz == 1) ? 0.0f : tt;
     }
   }
-  const float W = rd(incl, Z - 1);
-  if (!RC.valid || ev - RC.base >= 64) refill_rand(RC, ev, cid, E);
-  const int k = (int)(ev - RC.base);
-  const float u_sel = rdlane(RC.sel, k);
-  const float dt = E.exp_wait ? (-dlogf(rdlane(RC.wait, k))) / W : 1.0f / W;
-  int st = 0;
-  if (dt == 0.0f) st = 1;
-  if (dt != dt) st = 2;
-  if (__float_as_uint(dt) == 0x7f800000u) st = 3;
-  if (st) { stall = st; L.kind = kEvStall; return; }
-  t += (double)dt;
-  dt_prev = dt;
-  ev += 1;
-  const float pr = W * u_sel;
-  int p = -1;
+  const float Eend = (((cdcz + sdc1) + sdcz) + sdd1) + sddz;
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const unsigned long long m = __ballot(incl[s] > pr && (s * 64 + lane) < Z);
-    if (p < 0 && m) p = s * 64 + (__ffsll((long long)m) - 1);
-  }
-  if (p < 0) return;
-  float r = (p == 0) ? pr : pr - rd(incl, p - 1);
-  const float u_tau = rdlane(RC.tau, k);
-  const float g0 = rdlane(RC.g0, k), g1 = rdlane(RC.g1, k), g2 = rdlane(RC.g2, k);
-  if (p < Z - 1) {
-    const int e = p;
-    const int Ne = rd(C.n, e);
-    float sub[4];
-    sub[0] = E.cd ? rd(C.w, e) : 0.0f;
-    sub[1] = rd(C.wp, e);
-    sub[2] = rd(C.wm, e);
-    sub[3] = T.CC(Ne);
-    const int m = pick_sub(sub, 4, r);
-    if (m == 1 || m == 2) {          // SD shift
-      L.kind = (m == 1) ? kEvShiftP : kEvShiftM;
-      L.dn_idx = e; L.dn = (m == 1) ? 1 : -1;
-      L.lo = e - 1; L.hi = e + 1; L.dlo = e; L.dhi = e + 1;
-    } else if (m == 0) {             // CD destruction: merge strands e, e+1
-      const int Nm = Ne + rd(C.n, e + 1);
-      float mx = 0.0f, my = 0.0f, mz = 0.0f;
-      if (!(e == 0 || e == Z - 2)) {
-        mx = rd(C.qx, e) + rd(C.qx, e + 1);
-        my = rd(C.qy, e) + rd(C.qy, e + 1);
-        mz = rd(C.qz, e) + rd(C.qz, e + 1);
-      }
-      L.kind = kEvCDd; L.gap = -1; L.at = e + 1; L.znew = Z - 1;
-      L.p0 = e; L.p0N = Nm; L.p0x = mx; L.p0y = my; L.p0z = mz;
-      L.w0i = e; L.w0 = rd(C.w, e + 1);
-      L.lo = e - 1; L.hi = e; L.dlo = e; L.dhi = Z - 1;
-    } else if (m == 3) {             // CD creation inside strand e
-      const int newn = split_newn(r, Ne, sub[3]);
-      const float wnew = rate_fdt(E.P, u_tau);
-      L.kind = kEvCDc; L.gap = 1; L.at = e; L.znew = Z + 1;
-      L.w0i = e; L.w0 = wnew;
-      if (e == 0) {
-        const float sig = sqrtf((float)(Ne - newn) / 3.0f);
-        L.p0 = 0; L.p0N = newn;
-        L.p1 = 1; L.p1N = Ne - newn; L.p1x = g0 * sig; L.p1y = g1 * sig; L.p1z = g2 * sig;
-        L.lo = 0; L.hi = 1;
-      } else {
-        const float sig = sqrtf(((float)newn * (float)(Ne - newn)) / (3.0f * (float)Ne));
-        const float ratio = (float)newn / (float)Ne;
-        const float qx = rd(C.qx, e), qy = rd(C.qy, e), qz = rd(C.qz, e);
-        const float nx = (g0 * sig) + (qx * ratio);
-        const float ny = (g1 * sig) + (qy * ratio);
-        const float nz = (g2 * sig) + (qz * ratio);
-        L.p0 = e; L.p0N = newn; L.p0x = nx; L.p0y = ny; L.p0z = nz;
-        L.p1 = e + 1; L.p1N = Ne - newn; L.p1x = qx - nx; L.p1y = qy - ny; L.p1z = qz - nz;
-        L.lo = e - 1; L.hi = e + 1;
-      }
-      L.dlo = e; L.dhi = Z;
-    }
+  for (int j = 0; j < 4 * G; ++j) v[j] = (L.pos(j) == Z - 1) ? Eend : v[j];
+  // 2. scan, total, clock
+  float Gs[4 * G];
+  const float W = chain_scan<CPW, G>(v, Gs, Z, L);
+#pragma unroll
+  for (int g = 0; g < G; ++g) S.put4(kStG, g, L.pos0, Gs[4 * g], Gs[4 * g + 1], Gs[4 * g + 2], Gs[4 * g + 3]);
+  const float u_sel = seg_get<CPW>(RC.sel, k_rng, L);
+  float dt;
+  if (E.exp_wait) {
+    const float u_wait = seg_get<CPW>(RC.wait, k_rng, L);
+    dt = (-dlogf(u_wait)) / W;
   } else {
-    const int m = pick_sub(er, 5, r);
-    if (m == 0) {                    // CD creation in the tail strand
-      const int N = rd(C.n, Z - 1);
-      const int newn = split_newn(r, N, er[0]);
-      const float sig = (Z == 1) ? 0.0f : sqrtf((float)newn / 3.0f);
-      L.kind = kEvCDcTail; L.znew = Z + 1;
-      L.p0 = Z - 1; L.p0N = newn;
-      if (Z != 1) { L.p0x = g0 * sig; L.p0y = g1 * sig; L.p0z = g2 * sig; }
-      L.w0i = Z - 1; L.w0 = rate_fdt(E.P, u_tau);
-      L.p1 = Z; L.p1N = N - newn;
-      L.w1i = Z; L.w1 = 0.0f;
-      L.lo = Z - 2; L.hi = Z - 1; L.dlo = Z - 1; L.dhi = Z;
-    } else if (m == 1) {             // SD creation at the head
-      const int N0 = rd(C.n, 0);
-      const float wnew = rate_ft(E.P, u_tau);
-      const float sig = (Z == 1) ? 0.0f : sqrtf((float)(N0 - 1) / 3.0f);
-      L.kind = kEvSDcHead; L.gap = 1; L.at = 0; L.znew = Z + 1;
-      L.p0 = 0; L.p0N = 1;
-      L.w0i = 0; L.w0 = E.cd ? wnew : 0.0f;
-      L.p1 = 1; L.p1N = N0 - 1;
-      if (Z != 1) { L.p1x = g0 * sig; L.p1y = g1 * sig; L.p1z = g2 * sig; }
-      L.lo = 0; L.hi = 1; L.dlo = 0; L.dhi = Z;
-    } else if (m == 2) {             // SD creation at the tail
-      const int Nz = rd(C.n, Z - 1);
-      const float wnew = rate_ft(E.P, u_tau);
-      const float sig = (Z == 1) ? 0.0f : sqrtf((float)(Nz - 1) / 3.0f);
-      L.kind = kEvSDcTail; L.znew = Z + 1;
-      L.p0 = Z - 1; L.p0N = Nz - 1;
-      if (Z != 1) { L.p0x = g0 * sig; L.p0y = g1 * sig; L.p0z = g2 * sig; }
-      L.w0i = Z - 1; L.w0 = E.cd ? wnew : 0.0f;
-      L.p1 = Z; L.p1N = 1;
-      L.w1i = Z; L.w1 = 0.0f;
-      L.lo = Z - 2; L.hi = Z - 1; L.dlo = Z - 1; L.dhi = Z;
-    } else if (m == 3) {             // SD destruction at the head: strands 0,1 merge into an end
-      L.kind = kEvSDdHead; L.gap = -1; L.at = 0; L.znew = Z - 1;
-      L.p0 = 0; L.p0N = rd(C.n, 1) + 1;
-      L.lo = 0; L.hi = 0; L.dlo = 0; L.dhi = Z - 1;
-    } else if (m == 4) {             // SD destruction at the tail
-      L.kind = kEvSDdTail; L.znew = Z - 1;
-      L.p0 = Z - 2; L.p0N = rd(C.n, Z - 2) + 1;
-      L.w0i = Z - 2; L.w0 = 0.0f;
-      L.lo = Z - 3; L.hi = Z - 3; L.dlo = Z - 2; L.dhi = Z - 2;
-    }
+    dt = 1.0f / W;
   }
-}
-
-// Steps 4-5: apply a plan to the register-resident chain.
-template <int S>
-__device__ __forceinline__ void kmc_apply(Chain<S>& C, const Tabs& T, int cd, const Plan& L) {
-  const int lane = lane_id();
-  if (L.dn != 0) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const int i = s * 64 + lane;
-      C.n[s] += (i == L.dn_idx) ? L.dn : (i == L.dn_idx + 1 ? -L.dn : 0);
-    }
-  } else if (L.gap != 0) {
-    if (L.gap > 0) {
-      gap_open_arr<S>(C.qx, L.at, C.Z); gap_open_arr<S>(C.qy, L.at, C.Z); gap_open_arr<S>(C.qz, L.at, C.Z);
-      gap_open_arr<S>(C.q2, L.at, C.Z); gap_open_arr<S>(C.w, L.at, C.Z); gap_open_arr<S>(C.wp, L.at, C.Z);
-      gap_open_arr<S>(C.wm, L.at, C.Z); gap_open_arr<S>(C.rr, L.at, C.Z); gap_open_arr<S>(C.n, L.at, C.Z);
+  const float pr = W * u_sel;
+  const int p = pick_position<CPW, G>(Gs, pr, Z, L);
+  wave_lds_sync();
+  // 3. sub-event (per lane, replicated per chain)
+  const int pa = p < CAP - 2 ? p : CAP - 2;  // in-range position for the reads below
+  const float Gprev = S.getf(kStG, pa > 0 ? pa - 1 : 0);
+  const int Np = S.geti(kStN, pa), Np1 = S.geti(kStN, pa + 1);
+  const float wpp = S.getf(kStWP, pa), wmp = S.getf(kStWM, pa);
+  const float wcp = S.getf(kStW, pa), wcp1 = S.getf(kStW, pa + 1);
+  int kind = kEvNone;
+  float r = 0.0f;
+  if (act) {
+    int st = 0;
+    if (dt == 0.0f) st = 1;
+    if (dt != dt) st = 2;
+    if (__float_as_uint(dt) == 0x7f800000u) st = 3;
+    if (st) {
+      H.stall = st;
+      kind = kEvStall;
     } else {
-      gap_close_arr<S>(C.qx, L.at, C.Z); gap_close_arr<S>(C.qy, L.at, C.Z); gap_close_arr<S>(C.qz, L.at, C.Z);
-      gap_close_arr<S>(C.q2, L.at, C.Z); gap_close_arr<S>(C.w, L.at, C.Z); gap_close_arr<S>(C.wp, L.at, C.Z);
-      gap_close_arr<S>(C.wm, L.at, C.Z); gap_close_arr<S>(C.rr, L.at, C.Z); gap_close_arr<S>(C.n, L.at, C.Z);
+      H.t += (double)dt;
+      H.dt_prev = dt;
+      H.ev += 1;
+      H.zsum += (uint64_t)Z;
+      ++H.done;
+      --H.left;
+      if (p < Z) {
+        r = (p == 0) ? pr : pr - Gprev;
+        const bool pair = p < Z - 1;
+        const float s0 = pair ? (E.cd ? wcp : 0.0f) : cdcz;
+        const float s1 = pair ? wpp : sdc1;
+        const float s2 = pair ? wmp : sdcz;
+        const float s3 = pair ? T.CC(Np) : sdd1;
+        const float s4 = pair ? 0.0f : sddz;
+        // first m with s[m] > 0 and the running remainder below s[m]
+        int m = -1;
+        float rr = r;
+        if (s0 > 0.0f && rr < s0) { m = 0; } else { rr = rr - s0;
+        if (s1 > 0.0f && rr < s1) { m = 1; } else { rr = rr - s1;
+        if (s2 > 0.0f && rr < s2) { m = 2; } else { rr = rr - s2;
+        if (s3 > 0.0f && rr < s3) { m = 3; } else { rr = rr - s3;
+        if (s4 > 0.0f && rr < s4) { m = 4; } else { rr = rr - s4; } } } } }
+        r = rr;
+        if (pair) {
+          kind = m == 0 ? kEvCDd : m == 1 ? kEvShiftP : m == 2 ? kEvShiftM : m == 3 ? kEvCDc : kEvNone;
+        } else {
+          kind = m == 0 ? kEvCDcTail : m == 1 ? kEvSDcHead : m == 2 ? kEvSDcTail : m == 3 ? kEvSDdHead
+               : m == 4 ? kEvSDdTail : kEvNone;
+        }
+      }
     }
   }
-  C.Z = L.znew;
-  if (L.p0 >= 0 || L.w0i >= 0) {
-    const float q0 = q2_of(L.p0x, L.p0y, L.p0z), q1 = q2_of(L.p1x, L.p1y, L.p1z);
+  ++k_rng;
+  // 4. effect.  Common case: an N shift across pair p.
+  int lo = 1, hi = 0, zero_at = -1;
+  if (kind == kEvShiftP || kind == kEvShiftM) {
+    const int dn = (kind == kEvShiftP) ? 1 : -1;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const int i = s * 64 + lane;
-      if (i == L.p0) { C.n[s] = L.p0N; C.qx[s] = L.p0x; C.qy[s] = L.p0y; C.qz[s] = L.p0z; C.q2[s] = q0; }
-      if (i == L.p1) { C.n[s] = L.p1N; C.qx[s] = L.p1x; C.qy[s] = L.p1y; C.qz[s] = L.p1z; C.q2[s] = q1; }
-      if (i == L.w0i) C.w[s] = L.w0;
-      if (i == L.w1i) C.w[s] = L.w1;
+    for (int j = 0; j < 4 * G; ++j) {
+      const int d = L.pos(j) - p;
+      C.n[j] += (d == 0) ? dn : ((d == 1) ? -dn : 0);
+    }
+    lo = p - 1; hi = p + 1;
+    H.dlo = min(H.dlo, p); H.dhi = max(H.dhi, p + 1);
+  }
+  const bool rare = kind >= kEvCDd && kind <= kEvSDdTail;
+  if (__ballot(rare)) {
+    // values the rare effects read (cross-lane: run on the whole wave)
+    const int pq = p < CAP - 1 ? p : CAP - 2;
+    const float qxp = value_at<CPW, G>(C.qx, pq, L), qyp = value_at<CPW, G>(C.qy, pq, L);
+    const float qzp = value_at<CPW, G>(C.qz, pq, L);
+    const float qxp1 = value_at<CPW, G>(C.qx, pq + 1, L), qyp1 = value_at<CPW, G>(C.qy, pq + 1, L);
+    const float qzp1 = value_at<CPW, G>(C.qz, pq + 1, L);
+    const float u_tau = seg_get<CPW>(RC.tau, k_rng - 1, L);
+    const float g0 = seg_get<CPW>(RC.g0, k_rng - 1, L), g1 = seg_get<CPW>(RC.g1, k_rng - 1, L);
+    const float g2 = seg_get<CPW>(RC.g2, k_rng - 1, L);
+    const int N0n = S.geti(kStN, 0), N1n = S.geti(kStN, 1);
+    int gap = 0, at = CAP, znew = Z;
+    int p0 = -1, p0N = 0, p1 = -1, p1N = 0, w0i = -1, w1i = -1;
+    float p0x = 0.0f, p0y = 0.0f, p0z = 0.0f, p1x = 0.0f, p1y = 0.0f, p1z = 0.0f, w0 = 0.0f, w1 = 0.0f;
+    if (rare) {
+      const int e = p;
+      if (kind == kEvCDd) {            // CD destruction of link e: merge strands e, e+1
+        const int Ne = Np, Ne1 = Np1;
+        float mx = 0.0f, my = 0.0f, mz = 0.0f;
+        if (!(e == 0 || e == Z - 2)) { mx = qxp + qxp1; my = qyp + qyp1; mz = qzp + qzp1; }
+        gap = -1; at = e + 1; znew = Z - 1;
+        p0 = e; p0N = Ne + Ne1; p0x = mx; p0y = my; p0z = mz;
+        w0i = e; w0 = wcp1;
+        lo = e - 1; hi = e;
+        zero_at = Z - 2;  // new last position (the gap close shifts nothing when e = Z-2)
+        H.dlo = min(H.dlo, e); H.dhi = max(H.dhi, Z - 1);
+      } else if (kind == kEvCDc) {     // CD creation inside strand e
+        const int Ne = Np;
+        const float wc = T.CC(Ne);
+        const int newn = split_newn(r, Ne, wc);
+        const float wnew = rate_fdt(E.P, u_tau);
+        gap = 1; at = e; znew = Z + 1;
+        w0i = e; w0 = wnew;
+        if (e == 0) {
+          const float sig = sqrtf((float)(Ne - newn) / 3.0f);
+          p0 = 0; p0N = newn;
+          p1 = 1; p1N = Ne - newn; p1x = g0 * sig; p1y = g1 * sig; p1z = g2 * sig;
+          lo = 0; hi = 1;
+        } else {
+          const float sig = sqrtf(((float)newn * (float)(Ne - newn)) / (3.0f * (float)Ne));
+          const float ratio = (float)newn / (float)Ne;
+          const float nx = (g0 * sig) + (qxp * ratio);
+          const float ny = (g1 * sig) + (qyp * ratio);
+          const float nz = (g2 * sig) + (qzp * ratio);
+          p0 = e; p0N = newn; p0x = nx; p0y = ny; p0z = nz;
+          p1 = e + 1; p1N = Ne - newn; p1x = qxp - nx; p1y = qyp - ny; p1z = qzp - nz;
+          lo = e - 1; hi = e + 1;
+        }
+        H.dlo = min(H.dlo, e); H.dhi = max(H.dhi, Z);
+      } else if (kind == kEvCDcTail) { // CD creation in the tail strand
+        const int N = Nz;
+        const int newn = split_newn(r, N, cdcz);
+        const float sig = (Z == 1) ? 0.0f : sqrtf((float)newn / 3.0f);
+        znew = Z + 1;
+        p0 = Z - 1; p0N = newn;
+        if (Z != 1) { p0x = g0 * sig; p0y = g1 * sig; p0z = g2 * sig; }
+        w0i = Z - 1; w0 = rate_fdt(E.P, u_tau);
+        p1 = Z; p1N = N - newn;
+        w1i = Z; w1 = 0.0f;
+        lo = Z - 2; hi = Z - 1;
+        H.dlo = min(H.dlo, Z - 1); H.dhi = max(H.dhi, Z);
+      } else if (kind == kEvSDcHead) { // SD creation at the head
+        const float wnew = rate_ft(E.P, u_tau);
+        const float sig = (Z == 1) ? 0.0f : sqrtf((float)(N0n - 1) / 3.0f);
+        gap = 1; at = 0; znew = Z + 1;
+        p0 = 0; p0N = 1;
+        w0i = 0; w0 = E.cd ? wnew : 0.0f;
+        p1 = 1; p1N = N0n - 1;
+        if (Z != 1) { p1x = g0 * sig; p1y = g1 * sig; p1z = g2 * sig; }
+        lo = 0; hi = 1;
+        H.dlo = min(H.dlo, 0); H.dhi = max(H.dhi, Z);
+      } else if (kind == kEvSDcTail) { // SD creation at the tail
+        const float wnew = rate_ft(E.P, u_tau);
+        const float sig = (Z == 1) ? 0.0f : sqrtf((float)(Nz - 1) / 3.0f);
+        znew = Z + 1;
+        p0 = Z - 1; p0N = Nz - 1;
+        if (Z != 1) { p0x = g0 * sig; p0y = g1 * sig; p0z = g2 * sig; }
+        w0i = Z - 1; w0 = E.cd ? wnew : 0.0f;
+        p1 = Z; p1N = 1;
+        w1i = Z; w1 = 0.0f;
+        lo = Z - 2; hi = Z - 1;
+        H.dlo = min(H.dlo, Z - 1); H.dhi = max(H.dhi, Z);
+      } else if (kind == kEvSDdHead) { // SD destruction at the head: strands 0,1 merge into an end
+        gap = -1; at = 0; znew = Z - 1;
+        p0 = 0; p0N = N1n + 1;
+        lo = 0; hi = 0;
+        H.dlo = min(H.dlo, 0); H.dhi = max(H.dhi, Z - 1);
+      } else {                          // SD destruction at the tail
+        znew = Z - 1;
+        p0 = Z - 2; p0N = Nz2 + 1;
+        w0i = Z - 2; w0 = 0.0f;
+        lo = Z - 3; hi = Z - 3;
+        zero_at = Z - 2;
+        H.dlo = min(H.dlo, Z - 2); H.dhi = max(H.dhi, Z - 2);
+      }
+    }
+    // gaps (whole wave; identity on chains without one)
+    const bool any_open = __ballot(gap > 0) != 0, any_close = __ballot(gap < 0) != 0;
+    if (any_open) {
+      const int ao = gap > 0 ? at : CAP;
+      gap_open<CPW, G>(C.qx, ao, Z, L); gap_open<CPW, G>(C.qy, ao, Z, L); gap_open<CPW, G>(C.qz, ao, Z, L);
+      gap_open<CPW, G>(C.q2, ao, Z, L); gap_open<CPW, G>(C.w, ao, Z, L); gap_open<CPW, G>(C.n, ao, Z, L);
+      gap_lds<CPW, G>(S, 1, ao, Z, L);
+    }
+    if (any_close) {
+      const int ac = gap < 0 ? at : CAP;
+      gap_close<CPW, G>(C.qx, ac, Z, L); gap_close<CPW, G>(C.qy, ac, Z, L); gap_close<CPW, G>(C.qz, ac, Z, L);
+      gap_close<CPW, G>(C.q2, ac, Z, L); gap_close<CPW, G>(C.w, ac, Z, L); gap_close<CPW, G>(C.n, ac, Z, L);
+      gap_lds<CPW, G>(S, -1, ac, Z, L);
+    }
+    // patches
+    const float q0 = q2_of(p0x, p0y, p0z), q1 = q2_of(p1x, p1y, p1z);
+#pragma unroll
+    for (int j = 0; j < 4 * G; ++j) {
+      const int ps = L.pos(j);
+      if (ps == p0) { C.n[j] = p0N; C.qx[j] = p0x; C.qy[j] = p0y; C.qz[j] = p0z; C.q2[j] = q0; }
+      if (ps == p1) { C.n[j] = p1N; C.qx[j] = p1x; C.qy[j] = p1y; C.qz[j] = p1z; C.q2[j] = q1; }
+      if (ps == w0i) C.w[j] = w0;
+      if (ps == w1i) C.w[j] = w1;
+    }
+    H.Z = znew;
+    stage_qw<CPW, G>(S, C, L);
+  }
+  // 5. refresh the pairs next to the change
+  stage_n<CPW, G>(S, C, L);
+  wave_lds_sync();
+  refresh_pairs<CPW, G, LT>(S, T, E.cd, H.Z, lo, hi, zero_at, L);
+  wave_lds_sync();
+}
+
+// ------------------------------------------------------------------ load / store
+template <int CPW, int G>
+__device__ __forceinline__ void load_strands(Strands<G>& C, const ChainArrays& ch, const Hdr& H, int64_t base,
+                                             const LaneC<CPW>& L) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int p0 = 256 * g + L.pos0;
+    float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x, z = x, w = x;
+    int4 n = make_int4(0, 0, 0, 0);
+    if (!H.inert && p0 < H.Z) {
+      const int64_t o = base + p0;
+      x = *(const float4*)(ch.Qx + o); y = *(const float4*)(ch.Qy + o); z = *(const float4*)(ch.Qz + o);
+      w = *(const float4*)(ch.w + o); n = *(const int4*)(ch.N + o);
+    }
+    const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w}, zs[4] = {z.x, z.y, z.z, z.w};
+    const float ws[4] = {w.x, w.y, w.z, w.w};
+    const int ns[4] = {n.x, n.y, n.z, n.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int j = 4 * g + k;
+      const bool on = p0 + k < H.Z;
+      C.qx[j] = on ? xs[k] : 0.0f; C.qy[j] = on ? ys[k] : 0.0f; C.qz[j] = on ? zs[k] : 0.0f;
+      C.w[j] = on ? ws[k] : 0.0f; C.n[j] = on ? ns[k] : 0;
+      C.q2[j] = q2_of(C.qx[j], C.qy[j], C.qz[j]);
     }
   }
-  refresh<S>(C, T, cd, L.lo, L.hi);
 }
 
-struct Header {
-  int Z, nk, ens, stall, local;
-  uint32_t cid;
-  int64_t base, next_sample;
-  double t;
-  uint64_t ev, zsum;
-  float dt_prev;
-};
-
-__device__ __forceinline__ void load_header(Header& H, const ChainArrays& ch, int c) {
-  H.Z = ch.Z[c]; H.nk = ch.nk[c]; H.ens = ch.ens[c]; H.stall = ch.stall[c];
-  H.local = ch.local[c]; H.cid = ch.cid[c]; H.base = ch.base[c]; H.next_sample = ch.next_sample[c];
-  H.t = ch.t[c]; H.ev = ch.ev[c]; H.zsum = ch.zsum[c]; H.dt_prev = ch.dt_prev[c];
-}
-__device__ __forceinline__ void store_header(const Header& H, const ChainArrays& ch, int c) {
-  if (lane_id() == 0) {
-    ch.Z[c] = H.Z; ch.stall[c] = H.stall; ch.next_sample[c] = H.next_sample; ch.t[c] = H.t;
-    ch.ev[c] = H.ev; ch.zsum[c] = H.zsum; ch.dt_prev[c] = H.dt_prev;
-  }
-}
-
-template <int S>
-__device__ __forceinline__ void load_strands(Chain<S>& C, const ChainArrays& ch, int64_t base) {
-  const int lane = lane_id();
+// positions lo..hi (and < Z) back to HBM; whole 4-slot groups when they lie inside
+template <int CPW, int G>
+__device__ __forceinline__ void store_strands(const Strands<G>& C, const ChainArrays& ch, const Hdr& H, int64_t base,
+                                              int lo, int hi, const LaneC<CPW>& L) {
+  hi = min(hi, H.Z - 1);
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = s * 64 + lane;
-    if (i < C.Z) {
-      const int64_t o = base + i;
-      C.qx[s] = ch.Qx[o]; C.qy[s] = ch.Qy[o]; C.qz[s] = ch.Qz[o]; C.n[s] = ch.N[o]; C.w[s] = ch.w[o];
+  for (int g = 0; g < G; ++g) {
+    const int p0 = 256 * g + L.pos0;
+    const int64_t o = base + p0;
+    if (p0 >= lo && p0 + 3 <= hi) {
+      *(float4*)(ch.Qx + o) = make_float4(C.qx[4 * g], C.qx[4 * g + 1], C.qx[4 * g + 2], C.qx[4 * g + 3]);
+      *(float4*)(ch.Qy + o) = make_float4(C.qy[4 * g], C.qy[4 * g + 1], C.qy[4 * g + 2], C.qy[4 * g + 3]);
+      *(float4*)(ch.Qz + o) = make_float4(C.qz[4 * g], C.qz[4 * g + 1], C.qz[4 * g + 2], C.qz[4 * g + 3]);
+      *(float4*)(ch.w + o) = make_float4(C.w[4 * g], C.w[4 * g + 1], C.w[4 * g + 2], C.w[4 * g + 3]);
+      *(int4*)(ch.N + o) = make_int4(C.n[4 * g], C.n[4 * g + 1], C.n[4 * g + 2], C.n[4 * g + 3]);
     } else {
-      C.qx[s] = 0.0f; C.qy[s] = 0.0f; C.qz[s] = 0.0f; C.n[s] = 0; C.w[s] = 0.0f;
-    }
-    C.q2[s] = q2_of(C.qx[s], C.qy[s], C.qz[s]);
-    C.wp[s] = 0.0f; C.wm[s] = 0.0f; C.rr[s] = 0.0f;
-  }
-}
-
-template <int S>
-__device__ __forceinline__ void store_strands(const Chain<S>& C, const ChainArrays& ch, int64_t base, int lo,
-                                              int hi) {
-  const int lane = lane_id();
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = s * 64 + lane;
-    if (i >= lo && i <= hi && i < C.Z) {
-      const int64_t o = base + i;
-      ch.Qx[o] = C.qx[s]; ch.Qy[o] = C.qy[s]; ch.Qz[o] = C.qz[s]; ch.N[o] = C.n[s]; ch.w[o] = C.w[s];
+      for (int k = 0; k < 4; ++k) {
+        const int j = 4 * g + k;
+        if (p0 + k >= lo && p0 + k <= hi) {
+          ch.Qx[o + k] = C.qx[j]; ch.Qy[o + k] = C.qy[j]; ch.Qz[o + k] = C.qz[j];
+          ch.w[o + k] = C.w[j]; ch.N[o + k] = C.n[j];
+        }
+      }
     }
   }
 }
 
-// Table base for a chain of ensemble `ens`: the wave's LDS copy when the launch
-// stages tables in LDS (one copy per wave, refreshed when the ensemble changes),
-// otherwise the global arena.
-__device__ __forceinline__ Tabs wave_tables(const KmcArgs& a, const EnsParams& P, int ens, float* lds_wave,
-                                            int& lds_ens) {
-  Tabs T;
-  T.n = P.nmax + 2;
-  if (lds_wave) {
+// ------------------------------------------------------------------ a unit of work
+// Runs the chains of one work unit (four chains in row form, one in wave form) until
+// each reaches the ensemble's t_end (or its jump target ev_stop, or one jump in the
+// plain form), pushing chains that outgrow the geometry to the overflow list.
+template <int CPW, int G, bool LT, bool PLAIN>
+__device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg, lds_f* ltab, int& lds_ens) {
+  constexpr int CAP = Geo<CPW, G>::CAP;
+  constexpr int SEG = Geo<CPW, G>::SEG;
+  const LaneC<CPW> L;
+  const int c0 = __builtin_amdgcn_readfirstlane(a.work[CPW * unit]);
+  const int ens = __builtin_amdgcn_readfirstlane(a.ch.ens[c0]);
+  EnsLocal E;
+  load_ens(E, a.ens, ens);
+  Tabs<LT> T;
+  T.n = E.nmax + 2;
+  if constexpr (LT) {
     if (lds_ens != ens) {
-      const int tsz = 9 * T.n + 4 * P.nmax + 4;
-      const float* src = a.tabs + P.tab_off;
-      for (int i = lane_id(); i < tsz; i += 64) lds_wave[i] = src[i];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int tsz = 9 * T.n + 4 * E.nmax + 4;
+      const float* src = a.tabs + E.P->tab_off;
+      for (int i = L.lane; i < tsz; i += 64) ltab[i] = src[i];
+      wave_lds_sync();
       lds_ens = ens;
     }
-    T.b = lds_wave;
+    T.b = ltab;
   } else {
-    T.b = a.tabs + P.tab_off;
+    T.b = (typename TabPtr<false>::type)(a.tabs + E.P->tab_off);
   }
-  return T;
-}
-
-// Runs chain c until its clock passes the ensemble's t_end (or its jump counter
-// reaches ev_stop, or the chain outgrows 64*S strands).  Returns 1 on overflow.
-template <int S, bool PLAIN>
-__device__ __forceinline__ int run_chain(const KmcArgs& a, int c, float* lds_wave, int& lds_ens) {
-  Header H;
-  load_header(H, a.ch, c);
-  if (H.stall) return 0;
-  if (H.Z > 64 * S) return 1;
-  EnsLocal E;
-  load_ens(E, a.ens, H.ens);
-  const uint64_t ev_stop = a.use_ev_stop ? a.ch.ev_stop[c] : ~0ull;
-  // nothing to do: clock at target, no sample pending, no deformation pending
-  {
-    const bool pending_eq = E.sample_mode == kSampleEq && H.next_sample < E.k_end &&
-                            H.t > (double)H.next_sample * E.delta;
-    const bool pending_fl = E.sample_mode == kSampleFlow && H.next_sample < E.k_end &&
-                            H.t >= (double)H.next_sample * E.delta;
-    const bool pending_deform = E.flow && H.dt_prev != 0.0f;
-    if ((H.t >= E.t_end || H.ev >= ev_stop) && !pending_eq && !pending_fl && !pending_deform) return 0;
+  Hdr H;
+  H.c = (CPW == 4) ? a.work[CPW * unit + (L.lane >> 4)] : c0;
+  H.ovf = 0; H.done = 0; H.dlo = 1 << 30; H.dhi = -1;
+  if (H.c >= 0) {
+    const ChainArrays& ch = a.ch;
+    const int c = H.c;
+    H.Z = ch.Z[c]; H.nk = ch.nk[c]; H.stall = ch.stall[c]; H.local = ch.local[c]; H.cid = ch.cid[c];
+    H.next_sample = ch.next_sample[c]; H.t = ch.t[c]; H.ev = ch.ev[c];
+    H.zsum = ch.zsum[c]; H.dt_prev = ch.dt_prev[c];
+    const uint64_t stop = a.use_ev_stop ? ch.ev_stop[c] : ~0ull;
+    H.left = stop <= H.ev ? 0 : (stop - H.ev > 0x7fffffffull ? 0x7fffffff : (int)(stop - H.ev));
+    H.inert = H.stall != 0 || H.Z > CAP;
+    if (H.stall == 0 && H.Z > CAP) H.ovf = 1;
+  } else {
+    H.Z = 1; H.nk = 1; H.stall = 0; H.local = 0; H.cid = 0; H.next_sample = 0; H.t = 0.0;
+    H.ev = 0; H.zsum = 0; H.dt_prev = 0.0f; H.left = 0; H.inert = 1;
   }
-  const EnsParams& P = a.ens[H.ens];
-  const Tabs T = wave_tables(a, P, H.ens, lds_wave, lds_ens);
-  const float e1 = 1.0f / (P.beta * (float)H.nk);
-  Chain<S> C;
-  C.Z = H.Z;
-  load_strands<S>(C, a.ch, H.base);
-  refresh<S>(C, T, E.cd, 0, C.Z - 2);
-  RandCache RC;
-  RC.valid = false;
-  RC.base = 0;
-  int dlo = 1 << 30, dhi = -1;
-  int overflow = 0;
-  int done = 0;
+  const bool inert0 = H.inert != 0;
+  if (H.inert) H.Z = 1;
+  H.t_next = next_time(E, H.next_sample);
+  H.e1 = 1.0f / (E.beta * (float)H.nk);
+  Stage<CPW, G> S{stg, L.srow};
+  Strands<G> C;
+  load_strands<CPW, G>(C, a.ch, H, H.c >= 0 ? a.ch.base[H.c] : 0, L);
+  stage_n<CPW, G>(S, C, L);
+  stage_qw<CPW, G>(S, C, L);
+  refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+  wave_lds_sync();
+  RandCache RC{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int k_rng = SEG;
   for (;;) {
     if (E.flow) {
-      if (H.dt_prev != 0.0f) {
-        deform<S>(C, E.P, H.dt_prev);
-        refresh<S>(C, T, E.cd, 0, C.Z - 2);
-        dlo = 0; dhi = C.Z - 1;
+      const bool dfm = !H.inert && H.dt_prev != 0.0f;
+      if (__ballot(dfm)) {
+        if (dfm) deform<CPW, G>(C, E.P, H.dt_prev, H.Z, L);
+        stage_qw<CPW, G>(S, C, L);
+        refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+        wave_lds_sync();
+        if (dfm) { H.dlo = 0; H.dhi = max(H.dhi, H.Z - 1); }
       }
       H.dt_prev = 0.0f;
     }
-    record_samples<S>(C, T, E, H.t, H.next_sample, H.local, a.stage);
-    if (H.stall || H.t >= E.t_end || H.ev >= ev_stop) break;
-    if (PLAIN && done >= 1) break;
-    if (C.Z >= 64 * S) { overflow = 1; break; }
-    const int zb = C.Z;
-    Plan L;
-    kmc_pick<S>(C, T, E, e1, H.cid, RC, H.t, H.ev, H.dt_prev, H.stall, L);
-    if (L.kind == kEvStall) break;
-    H.zsum += (uint64_t)zb;
-    ++done;
-    if (L.kind != kEvNone) {
-      kmc_apply<S>(C, T, E.cd, L);
-      dlo = min(dlo, L.dlo);
-      dhi = max(dhi, L.dhi);
+    record_samples<CPW, G, LT>(C, T, E, H, a.stage, L);
+    bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0 && !(PLAIN && H.done >= 1);
+    if (act && H.Z >= CAP) { H.ovf = 1; act = false; }
+    H.inert = H.inert || (H.ovf != 0);
+    if (!__ballot(act)) break;
+    jump<CPW, G, LT>(C, H, act, S, T, E, RC, k_rng, L);
+  }
+  // write back every chain that was loaded live
+  if (H.c >= 0 && !inert0) {
+    const int64_t base = a.ch.base[H.c];
+    if (PLAIN) store_strands<CPW, G>(C, a.ch, H, base, H.dlo, H.dhi, L);
+    else store_strands<CPW, G>(C, a.ch, H, base, 0, H.Z - 1, L);
+    if (L.sl == 0) {
+      const int c = H.c;
+      a.ch.Z[c] = H.Z; a.ch.stall[c] = H.stall; a.ch.next_sample[c] = H.next_sample; a.ch.t[c] = H.t;
+      a.ch.ev[c] = H.ev; a.ch.zsum[c] = H.zsum; a.ch.dt_prev[c] = H.dt_prev;
     }
   }
-  H.Z = C.Z;
-  if (PLAIN) store_strands<S>(C, a.ch, H.base, dlo, dhi);
-  else store_strands<S>(C, a.ch, H.base, 0, C.Z - 1);
-  store_header(H, a.ch, c);
-  return overflow;
+  if (H.c >= 0 && H.ovf && L.sl == 0) a.overflow[atomicAdd(a.n_overflow, 1)] = H.c;
 }
 
-template <int S>
-__global__ void __launch_bounds__(256) kmc_batched_kernel(KmcArgs a) {
-  extern __shared__ float lds_tabs[];
-  float* lds_wave = a.lds_floats > 0 ? lds_tabs + (threadIdx.x >> 6) * a.lds_floats : nullptr;
+// LDS floats per wave: staging arrays, then the table copy
+template <int CPW, int G>
+__host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G>::STG; }
+
+// Persistent kernel: each wave takes work units from the queue until it is empty.
+// PLAIN: one jump per chain (the per-launch form); otherwise run to the targets.
+template <int CPW, int G, bool LT, bool PLAIN>
+__global__ void __launch_bounds__(64) kmc_kernel(KmcArgs a) {
+  extern __shared__ float lds_dyn[];
+  lds_f* stg = (lds_f*)lds_dyn;
+  lds_f* ltab = stg + stage_floats<CPW, G>();
   int lds_ens = -1;
+  const int n_units = (*a.n_work + CPW - 1) / CPW;
   for (;;) {
-    int item = 0;
-    if (lane_id() == 0) item = atomicAdd(a.queue, 1);
-    item = __builtin_amdgcn_readfirstlane(item);
-    if (item >= *a.n_work) break;
-    const int c = a.work[item];
-    if (run_chain<S, false>(a, c, lds_wave, lds_ens)) {
-      if (lane_id() == 0) a.overflow[atomicAdd(a.n_overflow, 1)] = c;
-    }
-  }
-}
-
-template <int S>
-__global__ void __launch_bounds__(256) kmc_plain_kernel(KmcArgs a) {
-  const int item = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  if (item >= *a.n_work) return;
-  int lds_ens = -1;
-  const int c = a.work[item];
-  if (run_chain<S, true>(a, c, nullptr, lds_ens)) {
-    if (lane_id() == 0) a.overflow[atomicAdd(a.n_overflow, 1)] = c;
+    int unit = 0;
+    if (lane_id() == 0) unit = atomicAdd(a.queue, 1);
+    unit = __builtin_amdgcn_readfirstlane(unit);
+    if (unit >= n_units) break;
+    run_unit<CPW, G, LT, PLAIN>(a, unit, stg, ltab, lds_ens);
   }
 }
 

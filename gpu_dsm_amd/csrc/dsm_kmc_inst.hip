@@ -1,35 +1,48 @@
-// dsm_kmc_inst.hip -- instantiates the KMC kernels for one register-slot count.
-// Built once per S (-DDSM_SLOTS=1,2,4,8,16) so the large unrolled variants
-// compile in parallel.
+// dsm_kmc_inst.hip -- instantiates the KMC kernels for one chain geometry.
+// Built once per geometry (-DDSM_GEOM=0..3) so the variants compile in parallel:
+//   0: row form, four chains per wave, Z <= 64
+//   1, 2, 3: wave form, one chain per wave, Z <= 256, 512, 1024
 #include "dsm_kmc.h"
 #include "dsm_launch.h"
 
-#ifndef DSM_SLOTS
-#error "DSM_SLOTS must be defined"
+#ifndef DSM_GEOM
+#error "DSM_GEOM must be defined"
 #endif
 #define DSM_CAT2(a, b) a##b
 #define DSM_CAT(a, b) DSM_CAT2(a, b)
 
 namespace dsm {
 
-hipError_t DSM_CAT(launch_kmc_slots_, DSM_SLOTS)(const KmcArgs& a, bool plain, int grid, hipStream_t st) {
-  if (grid <= 0) return hipSuccess;
-  if (plain) {
-    const int blocks = (int)(((int64_t)grid * 64 + 255) / 256);
-    hipLaunchKernelGGL(kmc_plain_kernel<DSM_SLOTS>, dim3(blocks), dim3(256), 0, st, a);
-  } else {
-    const size_t lds = (size_t)4 * a.lds_floats * sizeof(float);
-    hipLaunchKernelGGL(kmc_batched_kernel<DSM_SLOTS>, dim3(grid), dim3(256), lds, st, a);
-  }
+namespace {
+constexpr int kCPW = DSM_GEOM == 0 ? 4 : 1;
+constexpr int kG = DSM_GEOM == 0 ? 1 : (DSM_GEOM == 1 ? 1 : (DSM_GEOM == 2 ? 2 : 4));
+
+template <bool LT, bool PLAIN>
+hipError_t launch_one(const KmcArgs& a, int grid, hipStream_t st) {
+  const size_t lds = sizeof(float) * ((size_t)stage_floats<kCPW, kG>() + (LT ? (size_t)a.lds_floats : 0));
+  hipLaunchKernelGGL((kmc_kernel<kCPW, kG, LT, PLAIN>), dim3(grid), dim3(64), lds, st, a);
   return hipGetLastError();
 }
 
-int DSM_CAT(kmc_occupancy_slots_, DSM_SLOTS)(bool plain, int lds_floats) {
+template <bool LT, bool PLAIN>
+int occupancy_one(int lds_floats) {
   int nb = 0;
-  const size_t lds = (size_t)4 * lds_floats * sizeof(float);
-  const hipError_t e = plain ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_plain_kernel<DSM_SLOTS>, 256, 0)
-                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_batched_kernel<DSM_SLOTS>, 256, lds);
+  const size_t lds = sizeof(float) * ((size_t)stage_floats<kCPW, kG>() + (LT ? (size_t)lds_floats : 0));
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_kernel<kCPW, kG, LT, PLAIN>, 64, lds);
   return e == hipSuccess ? nb : 0;
+}
+}  // namespace
+
+hipError_t DSM_CAT(launch_kmc_geom_, DSM_GEOM)(const KmcArgs& a, bool plain, int grid, hipStream_t st) {
+  if (grid <= 0) return hipSuccess;
+  const bool lt = a.lds_floats > 0;
+  if (plain) return lt ? launch_one<true, true>(a, grid, st) : launch_one<false, true>(a, grid, st);
+  return lt ? launch_one<true, false>(a, grid, st) : launch_one<false, false>(a, grid, st);
+}
+
+int DSM_CAT(kmc_occupancy_geom_, DSM_GEOM)(bool plain, int lds_floats) {
+  if (plain) return lds_floats > 0 ? occupancy_one<true, true>(lds_floats) : occupancy_one<false, true>(0);
+  return lds_floats > 0 ? occupancy_one<true, false>(lds_floats) : occupancy_one<false, false>(0);
 }
 
 }  // namespace dsm

@@ -7,19 +7,22 @@
 
 namespace dsm {
 
-// KMC: S = register slots per lane (1,2,4,8,16,32); plain = one jump per chain
-// per launch, grid = number of work items; batched = persistent, grid = blocks.
-hipError_t launch_kmc(const KmcArgs& a, int S, bool plain, int grid, hipStream_t st);
-#define DSM_DECLARE_SLOTS(S)                                                                   \
-  hipError_t launch_kmc_slots_##S(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
-  int kmc_occupancy_slots_##S(bool plain, int lds_floats);
-DSM_DECLARE_SLOTS(1)
-DSM_DECLARE_SLOTS(2)
-DSM_DECLARE_SLOTS(4)
-DSM_DECLARE_SLOTS(8)
-DSM_DECLARE_SLOTS(16)
-#undef DSM_DECLARE_SLOTS
-int kmc_max_blocks_per_cu(int S, bool plain, int lds_floats);
+// KMC kernels by chain geometry: 0 = row form (four chains per wave, Z <= 64),
+// 1/2/3 = wave form (one chain per wave, Z <= 256/512/1024).  Persistent: grid =
+// blocks of one wave; plain = one jump per chain.  lds_floats > 0 stages the rate
+// tables (that many floats) in LDS.
+constexpr int kNumGeoms = 4;
+hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st);
+int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats);
+int kmc_geom_capacity(int geom);  // positions per chain
+#define DSM_DECLARE_GEOM(G)                                                                   \
+  hipError_t launch_kmc_geom_##G(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
+  int kmc_occupancy_geom_##G(bool plain, int lds_floats);
+DSM_DECLARE_GEOM(0)
+DSM_DECLARE_GEOM(1)
+DSM_DECLARE_GEOM(2)
+DSM_DECLARE_GEOM(3)
+#undef DSM_DECLARE_GEOM
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int* chains, int n, hipStream_t st);
 hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_iota(int* p, int start, int n, hipStream_t st);
@@ -34,7 +37,8 @@ hipError_t launch_corr_update(const float* stage, double* D, double* A, double* 
                               int nch, int p, int m, int s, int nrows, const int* cnt, hipStream_t st);
 hipError_t launch_corr_reduce(const double* C, const int* stall, int chain0, int nch, int npts, double* out_sum,
                               int* out_healthy, hipStream_t st);
-hipError_t launch_scan_probe(const float* in, float* out, int n, hipStream_t st);
+hipError_t launch_copy_probe(const void* in, void* out, size_t n4, int grid, hipStream_t st);
+hipError_t launch_scan_probe(const float* in, float* out, int n, int form, hipStream_t st);
 hipError_t launch_math_probe(const float* in, float* out, int n, int which, hipStream_t st);
 hipError_t launch_philox_probe(const uint32_t* ck, uint32_t* out, int n, hipStream_t st);
 

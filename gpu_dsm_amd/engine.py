@@ -23,7 +23,7 @@ EXPORTS = ["dsm_create", "dsm_destroy", "dsm_last_error", "dsm_mem_info", "dsm_a
            "dsm_remove_ensemble", "dsm_init_equilibrium", "dsm_state_cap", "dsm_get_state", "dsm_set_state",
            "dsm_advance", "dsm_advance_many", "dsm_advance_events", "dsm_flow_stress", "dsm_correlator",
            "dsm_last_samples", "dsm_tables", "dsm_ensemble_scalars", "dsm_probe_scan", "dsm_probe_math",
-           "dsm_probe_philox", "dsm_main", "dsm_main_ex"]
+           "dsm_probe_philox", "dsm_probe_copy", "dsm_main", "dsm_main_ex"]
 
 
 class DsmError(RuntimeError):
@@ -85,9 +85,10 @@ def load_library():
     L.dsm_last_samples.argtypes = [C.c_void_p, C.c_int32, P(C.c_float), C.c_int64, P(C.c_int64), P(C.c_int64)]
     L.dsm_tables.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_float), P(C.c_int32)]
     L.dsm_ensemble_scalars.argtypes = [C.c_void_p, C.c_int32, P(C.c_float)]
-    L.dsm_probe_scan.argtypes = [C.c_int32, P(C.c_float), P(C.c_float), C.c_int32]
+    L.dsm_probe_scan.argtypes = [C.c_int32, P(C.c_float), P(C.c_float), C.c_int32, C.c_int32]
     L.dsm_probe_math.argtypes = [C.c_int32, P(C.c_float), P(C.c_float), C.c_int32, C.c_int32]
     L.dsm_probe_philox.argtypes = [C.c_int32, P(C.c_uint32), P(C.c_uint32), C.c_int32]
+    L.dsm_probe_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, P(C.c_double)]
     L.dsm_main_ex.argtypes = [P(C.c_bool), C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_bool, P(C.c_int),
                               P(MainOptions)]
     _lib = L
@@ -267,11 +268,21 @@ class Engine:
 
 
 # ---------------------------------------------------------------- building-block probes
-def probe_scan(values, device=0):
+def probe_scan(values, form=1, device=0):
+    """Canonical scan tree on the device.  form 1: one chain (len <= 1024) -> (inclusive
+    sums, total).  form 0: values shaped (4, n<=64), four row-form chains -> (sums (4, n),
+    totals (4,))."""
     v = np.ascontiguousarray(values, np.float32)
-    out = np.zeros_like(v)
-    _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), len(v)))
-    return out
+    if form == 1:
+        n = v.size
+        out = np.zeros(n + 1, np.float32)
+        _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, 1))
+        return out[:n], out[n]
+    v = v.reshape(4, -1)
+    n = v.shape[1]
+    out = np.zeros((4, n + 1), np.float32)
+    _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, 0))
+    return out[:, :n], out[:, n]
 
 
 def probe_math(values, which, device=0):
@@ -287,6 +298,13 @@ def probe_philox(ctr_key, device=0):
     out = np.zeros((ck.shape[0], 4), np.uint32)
     _check(load_library().dsm_probe_philox(device, _p(ck, C.c_uint32), _p(out, C.c_uint32), ck.shape[0]))
     return out
+
+
+def probe_copy(n_bytes, reps=20, device=0):
+    """Milliseconds per plain device copy of n_bytes (read n_bytes + write n_bytes)."""
+    ms = C.c_double(0.0)
+    _check(load_library().dsm_probe_copy(device, int(n_bytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def run_main(job_id=0, savefile=None, loadfile=None, device=0, distr=False, mode="batched", corr_p=64, corr_m=8,

@@ -136,11 +136,18 @@ int dsm_last_samples(dsm_engine* e, int32_t ens, float* out, int64_t max_floats,
 int dsm_tables(dsm_engine* e, int32_t ens, int32_t which, float* out, int32_t* n);
 int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* out24);
 
-/* Building-block probes (run one kernel on the device): DPP wave scan over
-   64-position slots, exp/log/sin/cos (which 0..3), Philox (6 words in, 4 out). */
-int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n);
+/* Building-block probes (run one kernel on the device; no counterpart in the
+   reference): the canonical FP32 scan tree of the KMC kernels (form 1: one chain of
+   n <= 1024 positions, out = n inclusive sums + total; form 0: four chains of
+   n <= 64 positions in the row form, in = 4*n values, out = 4*(n+1)), exp/log/sin/cos
+   (which 0..3), Philox (6 words in, 4 out). */
+int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form);
 int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which);
 int dsm_probe_philox(int32_t device_id, const uint32_t* ctr_key, uint32_t* out, int32_t n);
+/* Plain device-to-device copy of n_bytes (multiple of 16), timed over `reps` launches
+   after one warm-up: the achievable-bandwidth and byte-counter reference (no
+   counterpart in the reference). */
+int dsm_probe_copy(int32_t device_id, int64_t n_bytes, int32_t reps, double* ms_per_copy);
 
 /* Options of the drop-in driver that input.dat cannot carry. */
 typedef struct {

@@ -20,7 +20,8 @@
  *     Kuhn-step boundary, which gives exactly the reference's joint law of Z
  *     (binomial) and N|Z (uniform compositions);
  *   - chain ends are identified by index (0 and Z-1) rather than by |Q| == 0;
- *   - the rate total is a wave64 tree sum in FP32 (the reference sums serially);
+ *   - rate totals are FP32 sums in a fixed tree order (the reference sums serially):
+ *     see the canonical scan tree below;
  *   - CD creation at the tail is gated by the CD flag in flow mode too.
  */
 #define _GNU_SOURCE
@@ -246,47 +247,62 @@ void orc_params_free(orc_params* p) {
 }
 
 /* ------------------------------------------------------------------ */
-/* wave64 scan tree (mirrors the DPP sequence of the HIP kernels):     */
-/*   row_shr:1,2,4,8 adds within rows of 16 (out-of-row source -> 0),  */
-/*   row_bcast:15 adds lane 15/47 to rows 1/3, row_bcast:31 adds lane   */
-/*   31 to rows 2,3; slots of 64 positions are chained by a carry.      */
+/* Canonical FP32 scan tree over positions 0..n-1 (zeros beyond n), the */
+/* summation order the HIP kernels use bit for bit.  Position           */
+/* p = 64b + 4l + k: block b of 64, lane l of 16, slot k of 4.          */
+/*   lane:  serial prefix s_k over the lane's 4 slots;                  */
+/*   block: Hillis-Steele inclusive scan H of the 16 lane totals        */
+/*          (strides 1,2,4,8; a source before lane 0 adds 0), exclusive */
+/*          prefix X_l = H_{l-1} (X_0 = 0), block-local I = X_l + s_k;  */
+/*   chain: G = I in block 0, G = C_b + I in block b >= 1, with         */
+/*          C_1 = I(63 of block 0), C_{b+1} = C_b + I(63 of block b).   */
+/* The total of n positions is G at the last position of the last       */
+/* occupied block (64*ceil(n/64) - 1).                                  */
 /* ------------------------------------------------------------------ */
-static void scan64(float v[64]) {
-  float t[64];
+static void block_scan(const float v[64], float I[64]) {
+  float s[64], H[16], t[16];
+  for (int l = 0; l < 16; ++l) {
+    float acc = v[4 * l];
+    s[4 * l] = acc;
+    for (int k = 1; k < 4; ++k) { acc = acc + v[4 * l + k]; s[4 * l + k] = acc; }
+    H[l] = acc;
+  }
   for (int d = 1; d <= 8; d <<= 1) {
-    for (int i = 0; i < 64; ++i) t[i] = ((i & 15) >= d) ? v[i - d] : 0.0f;
-    for (int i = 0; i < 64; ++i) v[i] = v[i] + t[i];
+    for (int l = 0; l < 16; ++l) t[l] = l >= d ? H[l - d] : 0.0f;
+    for (int l = 0; l < 16; ++l) H[l] = H[l] + t[l];
   }
-  for (int i = 0; i < 64; ++i) {
-    const int row = i >> 4;
-    t[i] = (row == 1) ? v[15] : (row == 3) ? v[47] : 0.0f;
+  for (int l = 0; l < 16; ++l) {
+    const float X = l == 0 ? 0.0f : H[l - 1];
+    for (int k = 0; k < 4; ++k) I[4 * l + k] = X + s[4 * l + k];
   }
-  for (int i = 0; i < 64; ++i) v[i] = v[i] + t[i];
-  for (int i = 0; i < 64; ++i) t[i] = (i >= 32) ? v[31] : 0.0f;
-  for (int i = 0; i < 64; ++i) v[i] = v[i] + t[i];
 }
 
-void orc_scan_tree(const float* v, int n, float* incl) {
-  float buf[64];
+/* writes G for positions 0..n-1; returns the total */
+static float scan_tree_total(const float* v, int n, float* incl) {
+  float buf[64], I[64];
   float carry = 0.0f;
-  for (int s = 0; s * 64 < n; ++s) {
-    for (int l = 0; l < 64; ++l) {
-      const int i = s * 64 + l;
-      buf[l] = i < n ? v[i] : 0.0f;
+  for (int b = 0; b * 64 < n; ++b) {
+    for (int q = 0; q < 64; ++q) {
+      const int i = b * 64 + q;
+      buf[q] = i < n ? v[i] : 0.0f;
     }
-    scan64(buf);
-    for (int l = 0; l < 64; ++l) {
-      const int i = s * 64 + l;
-      if (i < n) incl[i] = s == 0 ? buf[l] : carry + buf[l];
+    block_scan(buf, I);
+    for (int q = 0; q < 64; ++q) {
+      const int i = b * 64 + q;
+      if (i < n && incl) incl[i] = b == 0 ? I[q] : carry + I[q];
     }
-    carry = s == 0 ? buf[63] : carry + buf[63];
+    carry = b == 0 ? I[63] : carry + I[63];
   }
+  return carry;
 }
+
+void orc_scan_tree(const float* v, int n, float* incl) { (void)scan_tree_total(v, n, incl); }
+
+float orc_tree_total(const float* v, int n) { return n > 0 ? scan_tree_total(v, n, NULL) : 0.0f; }
 
 static float tree_total(const float* v, int n, float* scratch) {
-  if (n <= 0) return 0.0f;
-  orc_scan_tree(v, n, scratch);
-  return scratch[n - 1];
+  (void)scratch;
+  return orc_tree_total(v, n);
 }
 
 /* ------------------------------------------------------------------ */
@@ -459,8 +475,7 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
   if (Z > 4096) { heap = malloc(sizeof(float) * 2 * (size_t)(Z + 64)); posv = heap; inclv = heap + Z + 64; }
   for (int e = 0; e < Z - 1; ++e) posv[e] = c->R[e];
   posv[Z - 1] = end_rates(c, p, endr);
-  orc_scan_tree(posv, Z, inclv);
-  const float W = inclv[Z - 1];
+  const float W = scan_tree_total(posv, Z, inclv);
   uint32_t ctr[4] = {(uint32_t)c->ev, (uint32_t)(c->ev >> 32), c->chain_id, ORC_PURPOSE_STEP}, x[4];
   orc_philox4x32_10(ctr, key, x);
   const float dt = p->exp_wait ? (-orc_logf(orc_u01(x[1]))) / W : 1.0f / W;
@@ -675,8 +690,10 @@ int64_t orc_chain_advance(orc_chain* c, const orc_params* p, const uint32_t key[
       while (c->next_sample < k_end && c->t > (double)c->next_sample * delta) {
         float sv[3];
         orc_chain_stress_sample(c, p, sv);
-        float* o = samples + (c->next_sample - k0) * stride;
-        o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+        if (c->next_sample >= k0) {  /* rows k0..k_end-1 only, as the kernels */
+          float* o = samples + (c->next_sample - k0) * stride;
+          o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+        }
         c->next_sample++;
       }
     } else if (samples) {
@@ -684,8 +701,10 @@ int64_t orc_chain_advance(orc_chain* c, const orc_params* p, const uint32_t key[
       while (c->next_sample < k_end && c->t >= (double)c->next_sample * delta) {
         float sv[8];
         orc_chain_stress_full(c, p, sv);
-        float* o = samples + (c->next_sample - k0) * stride;
-        for (int q = 0; q < 8; ++q) o[q] = sv[q];
+        if (c->next_sample >= k0) {
+          float* o = samples + (c->next_sample - k0) * stride;
+          for (int q = 0; q < 8; ++q) o[q] = sv[q];
+        }
         c->next_sample++;
       }
     }
@@ -968,16 +987,20 @@ int64_t orc_chain64_advance(orc_chain64* c, const orc_params* p, const uint32_t 
       while (c->next_sample < k_end && c->t > (double)c->next_sample * delta) {
         double sv[3];
         orc_chain64_stress_sample(c, sv);
-        double* o = samples + (c->next_sample - k0) * stride;
-        o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+        if (c->next_sample >= k0) {
+          double* o = samples + (c->next_sample - k0) * stride;
+          o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+        }
         c->next_sample++;
       }
     } else if (samples) {
       while (c->next_sample < k_end && c->t >= (double)c->next_sample * delta) {
         double sv[8];
         orc_chain64_stress_full(c, sv);
-        double* o = samples + (c->next_sample - k0) * stride;
-        for (int q = 0; q < 8; ++q) o[q] = sv[q];
+        if (c->next_sample >= k0) {
+          double* o = samples + (c->next_sample - k0) * stride;
+          for (int q = 0; q < 8; ++q) o[q] = sv[q];
+        }
         c->next_sample++;
       }
     }

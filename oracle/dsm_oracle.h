@@ -9,7 +9,7 @@
  *
  * Two modes:
  *   - mirror-FP32: performs exactly the FP32 operations of the HIP kernels,
- *     in the same order (rate formula, wave64 scan tree, Philox streams,
+ *     in the same order (rate formula, canonical scan tree, Philox streams,
  *     software exp/log/sincos), so integer chain state and Q are bit-exact.
  *   - FP64: the same algorithm and the same Philox streams with every
  *     floating-point quantity in double and libm transcendentals; used to
@@ -111,8 +111,11 @@ void orc_chain_init_eq(orc_chain* c, const orc_params* p, const uint32_t key[2],
 void orc_chain_refresh(orc_chain* c, const orc_params* p);   /* recompute every cached rate */
 /* one KMC jump; returns event kind */
 int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]);
-/* mirror of the wave64 scan tree over positions 0..n-1; writes inclusive sums */
+/* canonical FP32 scan tree over positions 0..n-1 (blocks of 64 = 16 lanes x 4 slots,
+   serial carry between blocks); writes inclusive sums G.  orc_tree_total is G at the
+   last position of the last occupied block (zeros beyond n). */
 void orc_scan_tree(const float* v, int n, float* incl);
+float orc_tree_total(const float* v, int n);
 /* EQ sample: (sxy, syz, sxz) */
 void orc_chain_stress_sample(const orc_chain* c, const orc_params* p, float out[3]);
 /* flow stress: xx yy zz xy yz zx Lpp Z */

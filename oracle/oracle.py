@@ -46,6 +46,8 @@ def lib():
         L.orc_sincos2pif.argtypes = [C.c_float, f32p, f32p]
         L.orc_boxmuller.argtypes = [u32, u32, f32p, f32p]
         L.orc_scan_tree.argtypes = [f32p, C.c_int, f32p]
+        L.orc_tree_total.argtypes = [f32p, C.c_int]
+        L.orc_tree_total.restype = C.c_float
         L.orc_vec_math.argtypes = [C.c_int, f32p, f32p, C.c_int]
         L.orc_params_new.argtypes = [C.c_float, C.c_int, C.c_int, C.c_int, f32p, C.c_int]
         L.orc_params_new.restype = C.c_void_p
@@ -134,6 +136,12 @@ def scan_tree(v):
     out = np.zeros_like(v)
     lib().orc_scan_tree(_p(v, C.c_float), len(v), _p(out, C.c_float))
     return out
+
+
+def tree_total(v):
+    """Total of the canonical scan tree (G at the end of the last occupied 64-block)."""
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    return lib().orc_tree_total(_p(v, C.c_float), len(v))
 
 
 def corr_levels(sim_time, delta, p, m):

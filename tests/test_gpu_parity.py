@@ -51,15 +51,23 @@ def test_philox_matches_rocrand_vectors():
     assert np.array_equal(out, np.array([v["out"] for v in vec], np.uint32))
 
 
-def test_wave_scan_tree_matches_oracle(oracle):
+def test_scan_tree_matches_oracle(oracle):
+    """Both chain geometries of the KMC kernels sum in the oracle's canonical tree."""
     import gpu_dsm_amd as g
     rng = np.random.default_rng(5)
-    for vals in (rng.random(64 * 7).astype(np.float32),
-                 (rng.standard_normal(64 * 5) * 10).astype(np.float32),
-                 rng.exponential(size=64 * 3).astype(np.float32) * np.float32(1e3)):
-        gpu = g.probe_scan(vals)
-        ref = np.concatenate([oracle.scan_tree(vals[i:i + 64]) for i in range(0, len(vals), 64)])
-        assert np.array_equal(_bits(gpu), _bits(ref))
+    for n in (1, 3, 17, 64, 65, 200, 256, 257, 640, 1000, 1024):
+        for vals in (rng.random(n).astype(np.float32),
+                     (rng.standard_normal(n) * 10).astype(np.float32),
+                     rng.exponential(size=n).astype(np.float32) * np.float32(1e3)):
+            gpu, tot = g.probe_scan(vals, form=1)
+            assert np.array_equal(_bits(gpu), _bits(oracle.scan_tree(vals)))
+            assert _bits(np.float32(tot)) == _bits(np.float32(oracle.tree_total(vals)))
+    for n in (1, 5, 40, 63, 64):
+        vals = (rng.standard_normal((4, n)) * 3).astype(np.float32)
+        gpu, tot = g.probe_scan(vals, form=0)
+        for r in range(4):
+            assert np.array_equal(_bits(gpu[r]), _bits(oracle.scan_tree(vals[r])))
+            assert _bits(np.float32(tot[r])) == _bits(np.float32(oracle.tree_total(vals[r])))
 
 
 @pytest.mark.parametrize("which", ["exp", "log", "sin2pi", "cos2pi"])
@@ -100,9 +108,14 @@ def test_init_parity(engine, oracle, case):
 KMC_CASES = [
     dict(n=300, nk=25, beta=1.0, cd=0, events=400),
     dict(n=200, nk=100, beta=1.0, cd=1, events=300),
-    dict(n=128, nk=50, beta=0.5, cd=1, events=300),
+    dict(n=127, nk=50, beta=0.5, cd=1, events=300),
     dict(n=96, nk=160, beta=1.0, cd=1, events=250),
     dict(n=48, nk=300, beta=1.0, cd=1, events=200),
+    dict(n=9, nk=1000, beta=0.5, cd=1, events=60),
+    # thousands of work units: several per persistent wave, and chains that reach the
+    # row form's capacity (Z = 64) mid-pass and continue in the wave form
+    dict(n=4096, nk=100, beta=1.0, cd=1, events=80),
+    dict(n=16384, nk=40, beta=1.0, cd=1, events=40),
     dict(n=128, nk=40, beta=2.0, cd=1, events=300, exp_wait=True),
     dict(n=128, nk=30, beta=1.0, cd=1, events=300, kappa=[0, 0.05, 0, 0, 0, 0, 0, 0, 0]),
     dict(n=64, nk=20, beta=1.0, cd=1, events=300, kappa=[0.02, 0, 0, 0, -0.01, 0, 0, 0, -0.01]),
@@ -191,3 +204,33 @@ def test_flow_stress_rows(engine, oracle):
     np.testing.assert_allclose(rows[:, :8], ref, rtol=1e-12, atol=1e-9)
     assert np.all(rows[:, 8] == n)
     engine.remove_ensemble(ens)
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_multi_ensemble_flow_batch(engine, oracle, mode):
+    """Several ensembles (different shear rates, chain counts that are not multiples of
+    four, one equilibrium ensemble) advanced together in one pass: each must match its
+    own oracle run bit for bit, with its own flow output rows."""
+    nk, T, dlt = 24, 12.0, 3.0
+    rates = [0.0, 0.01, 0.1, 0.5]
+    counts = [37, 64, 5, 18]
+    ids, refs = [], []
+    for q, (gd, n) in enumerate(zip(rates, counts)):
+        kappa = [0, gd, 0, 0, 0, 0, 0, 0, 0]
+        P = oracle.Params(1.0, nk, 1, kappa=kappa)
+        ens = engine.add_ensemble(n, nk, 1.0, 1, kappa=kappa, seed=40 + q, salt=q, sample="flow", delta=dlt,
+                                  window=3)
+        engine.init_equilibrium(ens)
+        O = oracle.Ensemble(P, seed=40 + q, n=n, salt=q)
+        k_end = int(T / dlt) + 1
+        _, samples = O.advance(t_end=T, delta=dlt, k0=1, k_end=k_end, samples=True, flow_samples=True)
+        ids.append(ens)
+        refs.append((O, samples, n, k_end))
+    engine.advance_many(ids, T, mode=mode)
+    for ens, (O, samples, n, k_end) in zip(ids, refs):
+        assert_same_state(engine.state(ens), O.state(), "ensemble %d" % ens)
+        rows = engine.flow_stress(ens)
+        assert rows.shape[0] == k_end - 1
+        np.testing.assert_allclose(rows[:, :8], samples.astype(np.float64).sum(axis=1), rtol=1e-12, atol=1e-9)
+    for ens in ids:
+        engine.remove_ensemble(ens)

@@ -292,3 +292,19 @@ def test_fp64_mode_agrees_statistically(oracle):
     za, zb = a.state()["Z"].mean(), b.state()["Z"].mean()
     assert za == pytest.approx(zb, abs=0.25)
     assert (sa.astype(np.float64) ** 2).mean() == pytest.approx((sb ** 2).mean(), rel=0.05)
+
+
+def test_flow_samples_stay_inside_the_window(oracle):
+    """A flow-output ensemble without shear starts its sample counter before the window;
+    samples before k0 are consumed without being written (guard rows stay untouched)."""
+    import ctypes as C
+    P = oracle.Params(1.0, 24, 1, kappa=[0] * 9)
+    E = oracle.Ensemble(P, seed=40, n=37, salt=0)
+    k0, k_end, n = 1, 5, 37
+    buf = np.full((k_end - k0 + 2, n, 8), np.float32(-7.0), np.float32)   # one guard row each side
+    oracle.lib().orc_params_set_flow_samples(P.h, 1)
+    inner = buf[1:-1]
+    assert inner.flags["C_CONTIGUOUS"]
+    oracle.lib().orc_ens_advance(E.h, 12.0, 2 ** 62, 3.0, k0, k_end, inner.ctypes.data_as(C.c_void_p), 2)
+    assert np.all(buf[0] == -7.0) and np.all(buf[-1] == -7.0)
+    assert np.all(inner[:, :, 7] >= 1)          # every row written (Z column)

@@ -119,3 +119,33 @@ def test_scan_tree_is_prefix_sum(oracle):
     z[37] = 1.5
     inc = oracle.scan_tree(z)
     assert np.all(inc[:37] == 0) and np.all(inc[37:] == 1.5)
+
+
+def test_scan_tree_exact_on_integers_and_total(oracle):
+    """Small integers sum exactly in any order: every inclusive sum and the total are
+    the exact prefix sums, over block boundaries (64) and ragged tails."""
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 4, 5, 63, 64, 65, 127, 128, 129, 256, 300, 1024):
+        v = rng.integers(0, 1000, size=n).astype(np.float32)
+        incl = oracle.scan_tree(v)
+        assert np.array_equal(incl, np.cumsum(v.astype(np.int64)).astype(np.float32))
+        assert oracle.tree_total(v) == float(v.astype(np.int64).sum())
+
+
+def test_scan_tree_order_is_blockwise(oracle):
+    """The association is lane-serial / 16-lane Hillis-Steele / block-serial: a value
+    pattern whose FP32 sum depends on the order separates it from a plain running sum."""
+    big, small = np.float32(2 ** 24), np.float32(1.0)
+    # running sum: big + 1 + 1 + 1 = big (each +1 rounds away); lane-serial over
+    # slots 0..3 of one lane gives the same; but Hillis-Steele adds lane 1's own
+    # (1+1+1+1 = 4) to lane 0's total, which is exact.
+    v = np.array([big, 0, 0, 0, small, small, small, small], np.float32)
+    incl = oracle.scan_tree(v)
+    run = np.float32(0)
+    seq = []
+    for x in v:
+        run = np.float32(run + x)
+        seq.append(run)
+    assert seq[-1] == big            # sequential sum loses the small terms
+    assert incl[-1] == big + 4       # the tree adds them up first
+    assert incl[4] == big            # inside lane 1: X + s_0 = big + 1 rounds to big
