@@ -10,8 +10,9 @@ launch) is measured on the same state beside it.
 Printed: one JSON line.  value = whole-job jumps/s of the batched form with the chain
 state already resident in HBM.  roofline uses the algorithmic bytes per jump
 B(Z) = 20*Z + 32 (strand records Q,N,w plus the chain header) over the device time of
-the dominant kernel (the single-slot batched KMC kernel), measured with HIP events on
-the engine's stream.  cpu_baseline is the CPU oracle (same algorithm, same Philox
+the dominant kernel (the row-form batched KMC kernel, four chains per wave), measured
+with HIP events on the engine's stream; roofline.traffic is that kernel's measured HBM
+bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated on a plain copy).  cpu_baseline is the CPU oracle (same algorithm, same Philox
 streams) timed on this host on a bounded chain subset.
 """
 import argparse
@@ -43,7 +44,7 @@ def parse():
 
 def traffic_from_profile():
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "kmc_batched_s1_c2_traffic.json")
+    path = os.path.join(ROOT, "profiles", "kmc_row_form_c2_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -146,7 +147,8 @@ def main():
         "config": {"workload": "monodisperse equilibrium melt, %d chains, Nk=%d, beta=%g, CD on, G(t) multi-tau "
                                "p=%d m=%d, %d tau_K per step" % (args.chains, w["nk"], w["beta"], w["corr_p"],
                                                                  w["corr_m"], args.window),
-                   "form": "batched persistent (Z-binned work queue, register-resident chains)",
+                   "form": "batched persistent (row form: four register-resident chains per wave; "
+                           "wave form for Z > 63)",
                    "n_chains": args.chains, "nk": w["nk"], "beta": w["beta"], "cd": w["cd"],
                    "window_tau_k": args.window},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
