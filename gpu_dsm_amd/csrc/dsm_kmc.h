@@ -244,10 +244,12 @@ __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr,
     int cand = CAP;
 #pragma unroll
     for (int k = 3; k >= 0; --k) cand = (Gv[k] > pr && L.pos0 + k < Z) ? L.pos0 + k : cand;
-    cand = min(cand, dppi<kRowShr1>(cand, CAP));
-    cand = min(cand, dppi<kRowShr2>(cand, CAP));
-    cand = min(cand, dppi<kRowShr4>(cand, CAP));
-    cand = min(cand, dppi<kRowShr8>(cand, CAP));
+    // row minimum (sources before the row's lane 0 contribute the identity)
+    constexpr int kIdent = 0x7fffffff;
+    cand = min(cand, dppi<kRowShr1>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr2>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr4>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr8>(cand, kIdent));
     return row_last(cand);
   } else {
     int res = CAP;
@@ -460,17 +462,43 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
   }
 }
 
-// Recompute the cached rates of pairs lo..hi (at most three, clamped to 0..Z-2) in
-// lanes 0..2 of the segment; lane 3 zeroes the rate cache at zero_at (>= 0).
+// Recompute the cached rates of pairs lo..hi (at most three, clamped to 0..Z-2).
+// Each pair is split over two lanes of the segment's first row running one
+// instruction stream: lane j (0..2) evaluates W+ of pair lo+j and lane 8+j its W-
+// (the same FP operations as pair_rate, with the tables and signs chosen per lane);
+// lane j then adds the halves (W- arrives by DPP row_ror:8) and writes W+, W-, R.
+// Lane 3 zeroes the rate cache at zero_at (>= 0).
 template <int CPW, int G, bool LT>
 __device__ __forceinline__ void refresh_pairs(const Stage<CPW, G>& S, const Tabs<LT>& T, int cd, int Z, int lo, int hi,
                                               int zero_at, const LaneC<CPW>& L) {
-  const int q = lo + L.sl;
-  if (L.sl < 3 && q <= hi && q >= 0 && q <= Z - 2) {
-    const int Ni = S.geti(kStN, q), Nj = S.geti(kStN, q + 1);
-    const float q2i = S.getf(kStQ2, q), q2j = S.getf(kStQ2, q + 1), we = S.getf(kStW, q);
-    float wp, wm, R;
-    pair_rate<LT>(T, cd, Ni, Nj, q2i, q2j, we, q == 0, q + 1 == Z - 1, wp, wm, R);
+  const int h = (L.sl >> 3) & 1;  // 0: W+, 1: W-
+  const int jl = L.sl & 7;
+  const int q = lo + jl;
+  const bool on = L.sl < 16 && jl < 3 && q <= hi && q >= 0 && q <= Z - 2;
+  float part = 0.0f, we = 0.0f;
+  int Ni = 0;
+  if (on) {
+    Ni = S.geti(kStN, q);
+    const int Nj = S.geti(kStN, q + 1);
+    const float q2i = S.getf(kStQ2, q), q2j = S.getf(kStQ2, q + 1);
+    we = S.getf(kStW, q);
+    const bool endi = q == 0, endj = q + 1 == Z - 1;
+    const int n = T.n;
+    if (h ? Ni > 1 : Nj > 1) {
+      // W+: a = q2i SP(Ni), b = q2j SM(Nj), FR(x) PP(Ni) PM(Nj), exp(a - b)
+      // W-: a = q2i SM(Ni), b = q2j SP(Nj), FR(x) PM(Ni) PP(Nj), exp(b - a)
+      const float a = q2i * T.at((h ? n : 0) + Ni);
+      const float b = q2j * T.at((h ? 0 : n) + Nj);
+      const int x = (endi ? 4 * Ni + (h ? -1 : 1) : 2 * Ni) + (endj ? 4 * Nj + (h ? 1 : -1) : 2 * Nj);
+      const float pf = (T.FR(x) * (endi ? 1.0f : T.at((h ? 3 : 2) * n + Ni))) *
+                       (endj ? 1.0f : T.at((h ? 2 : 3) * n + Nj));
+      part = pf * dexpf(h ? b - a : a - b);
+    }
+  }
+  const float wm = dppf<0x128>(part, 0.0f);  // row_ror:8: lane j receives lane 8+j
+  if (on && h == 0) {
+    const float wp = part;
+    const float R = cd ? (wp + wm) + (we + T.CC(Ni)) : (wp + wm);
     S.putf(kStWP, q, wp);
     S.putf(kStWM, q, wm);
     S.putf(kStRR, q, R);
