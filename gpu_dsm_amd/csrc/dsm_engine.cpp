@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -114,6 +115,54 @@ int bin_for(int nmax) {  // smallest geometry with capacity > nmax (room for one
 // strand slots per chain: Nk rounded up to whole 16-byte groups (aligned vector access)
 inline int cap_of(int nk) { return (nk + 3) & ~3; }
 
+// First-fit allocator over an index space [0, end) (chains, strand slots, staging
+// floats): the ranges of removed ensembles are handed out again, so a long-running
+// engine (serving) does not grow without bound.
+struct RangeAlloc {
+  std::map<int64_t, int64_t> free_;  // start -> length
+  int64_t end = 0;
+  int64_t take(int64_t n) {
+    if (n <= 0) return end;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        const int64_t s0 = it->first, len = it->second;
+        free_.erase(it);
+        if (len > n) free_[s0 + n] = len - n;
+        return s0;
+      }
+    }
+    if (!free_.empty()) {  // a free range at the end is extended
+      auto last = std::prev(free_.end());
+      if (last->first + last->second == end) {
+        const int64_t s0 = last->first;
+        free_.erase(last);
+        end = s0 + n;
+        return s0;
+      }
+    }
+    const int64_t s0 = end;
+    end += n;
+    return s0;
+  }
+  void give(int64_t s0, int64_t n) {
+    if (n <= 0) return;
+    auto it = free_.emplace(s0, n).first;
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+        it = prev;
+      }
+    }
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+  }
+};
+
 template <typename T>
 int dev_alloc(T** p, size_t n) {
   *p = nullptr;
@@ -167,6 +216,7 @@ struct dsm_engine {
   size_t d_tabs_cap = 0;
   std::vector<int64_t> stage_off;  // per ensemble
   int64_t stage_used = 0;
+  RangeAlloc chain_space, strand_space, stage_space;
   float* d_stage = nullptr;
   int64_t d_stage_cap = 0;
   // work lists: [bin][nch] + counts
@@ -259,9 +309,8 @@ int upload_tabs(dsm_engine* e) {
 }
 
 int ensure_stage(dsm_engine* e) {
-  if (e->stage_used > e->d_stage_cap) {
-    if (e->d_stage) HIPCHK(hipFree(e->d_stage));
-    int rc = dev_alloc(&e->d_stage, (size_t)e->stage_used);
+  if (e->stage_used > e->d_stage_cap) {  // keeps the rows of live ensembles (last_samples)
+    int rc = grow(&e->d_stage, e->d_stage_cap, e->stage_used, e->st);
     if (rc) return rc;
     e->d_stage_cap = e->stage_used;
   }
@@ -704,16 +753,18 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   E.m = d->corr_m > 0 ? d->corr_m : 8;
   if (E.p % E.m != 0 && d->sample_mode == DSM_SAMPLE_GT) return fail(DSM_ERR_ARG, "corr_p must be a multiple of corr_m");
   E.stage_floats = (int64_t)E.window * E.n * E.ncomp;
-  P.stage_off = e->stage_used;
-  e->stage_used += E.stage_floats;
+  P.stage_off = e->stage_space.take(E.stage_floats);
+  e->stage_used = std::max(e->stage_used, e->stage_space.end);
   if ((rc = ensure_stage(e))) return rc;
   // chains
-  E.chain0 = e->nch;
-  E.strand0 = e->nstr;
+  E.chain0 = (int)e->chain_space.take(E.n);
   E.nstrand = 0;
   for (int i = 0; i < E.n; ++i) E.nstrand += cap_of(E.nk[i]);
   P.chain0 = E.chain0;
-  if ((rc = grow_chains(e, e->nch + E.n, e->nstr + E.nstrand))) return rc;
+  E.strand0 = e->strand_space.take(E.nstrand);
+  if ((rc = grow_chains(e, (int)std::max<int64_t>(e->nch, e->chain_space.end),
+                        std::max<int64_t>(e->nstr, e->strand_space.end))))
+    return rc;
   {
     const int id = (int)e->ens.size();
     std::vector<int> hnk(E.n), hens(E.n, id), hcap(E.n), hzero(E.n, 0), hloc(E.n);
@@ -754,6 +805,11 @@ int dsm_remove_ensemble(dsm_engine* e, int32_t ens) {
   if (E->dD) { (void)hipFree(E->dD); (void)hipFree(E->dA); (void)hipFree(E->dC); E->dD = E->dA = E->dC = nullptr; }
   E->live = false;
   E->flow_rows.clear();
+  E->flow_rows.shrink_to_fit();
+  // its chains, strand slots and staging rows go back to the engine's free ranges
+  e->chain_space.give(E->chain0, E->n);
+  e->strand_space.give(E->strand0, E->nstrand);
+  e->stage_space.give(E->P.stage_off, E->stage_floats);
   return DSM_OK;
 }
 

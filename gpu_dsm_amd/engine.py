@@ -23,7 +23,7 @@ EXPORTS = ["dsm_create", "dsm_destroy", "dsm_last_error", "dsm_mem_info", "dsm_a
            "dsm_remove_ensemble", "dsm_init_equilibrium", "dsm_state_cap", "dsm_get_state", "dsm_set_state",
            "dsm_advance", "dsm_advance_many", "dsm_advance_events", "dsm_flow_stress", "dsm_correlator",
            "dsm_last_samples", "dsm_tables", "dsm_ensemble_scalars", "dsm_probe_scan", "dsm_probe_math",
-           "dsm_probe_philox", "dsm_probe_copy", "dsm_main", "dsm_main_ex"]
+           "dsm_probe_philox", "dsm_probe_copy", "dsm_main", "dsm_main_ex", "dsm_serve", "dsm_serve_plan"]
 
 
 class DsmError(RuntimeError):
@@ -44,6 +44,32 @@ class RunStats(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("n_requests", C.c_int32), ("arrival_rate", C.c_double), ("seed", C.c_uint64),
+                ("nk_min", C.c_int32), ("nk_max", C.c_int32), ("nk_step", C.c_int32), ("n_beta", C.c_int32),
+                ("beta", C.c_float * 4), ("nch_min", C.c_int32), ("nch_max", C.c_int32),
+                ("flow_fraction", C.c_double), ("rate_min", C.c_double), ("rate_max", C.c_double),
+                ("horizon", C.c_double), ("window", C.c_double), ("cd", C.c_int32), ("hbm_fraction", C.c_double)]
+
+
+class ServeReport(C.Structure):
+    _fields_ = [("completed", C.c_int32), ("max_active", C.c_int32), ("rounds", C.c_int32),
+                ("makespan_s", C.c_double), ("p50_s", C.c_double), ("p99_s", C.c_double), ("mean_s", C.c_double),
+                ("max_s", C.c_double), ("jumps", C.c_uint64), ("jumps_per_s", C.c_double),
+                ("peak_bytes", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# default serving stream: 2000 requests at 20/s, Nk in 16..512, beta in {0.5, 1, 2},
+# 1024..65536 chains, half of them sheared at 1e-5..1e-2 per tau_K, 1e5 tau_K each
+# (about 1.7e14 jumps in total; callers pass a shorter horizon for timed runs)
+SERVE_DEFAULTS = dict(n_requests=2000, arrival_rate=20.0, seed=1, nk_min=16, nk_max=512, nk_step=16,
+                      beta=(0.5, 1.0, 2.0), nch_min=1024, nch_max=65536, flow_fraction=0.5, rate_min=1e-5,
+                      rate_max=1e-2, horizon=1.0e5, window=10.0, cd=1, hbm_fraction=0.8)
 
 
 class MainOptions(C.Structure):
@@ -89,6 +115,9 @@ def load_library():
     L.dsm_probe_math.argtypes = [C.c_int32, P(C.c_float), P(C.c_float), C.c_int32, C.c_int32]
     L.dsm_probe_philox.argtypes = [C.c_int32, P(C.c_uint32), P(C.c_uint32), C.c_int32]
     L.dsm_probe_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, P(C.c_double)]
+    L.dsm_serve.argtypes = [C.c_int32, P(ServeConfig), P(ServeReport), P(C.c_double)]
+    L.dsm_serve_plan.argtypes = [P(ServeConfig), C.c_int32, P(C.c_double), P(C.c_int32), P(C.c_float),
+                                 P(C.c_int32), P(C.c_double), P(C.c_int64)]
     L.dsm_main_ex.argtypes = [P(C.c_bool), C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_bool, P(C.c_int),
                               P(MainOptions)]
     _lib = L
@@ -305,6 +334,42 @@ def probe_copy(n_bytes, reps=20, device=0):
     ms = C.c_double(0.0)
     _check(load_library().dsm_probe_copy(device, int(n_bytes), int(reps), C.byref(ms)))
     return ms.value
+
+
+def _serve_config(**kw):
+    c = dict(SERVE_DEFAULTS)
+    c.update(kw)
+    cfg = ServeConfig()
+    for k, v in c.items():
+        if k == "beta":
+            b = list(v)
+            cfg.n_beta = len(b)
+            for i, x in enumerate(b):
+                cfg.beta[i] = float(x)
+        else:
+            setattr(cfg, k, v)
+    return cfg, c
+
+
+def serve_plan(**kw):
+    """The request stream of a serving run (host only): dict of per-request arrays."""
+    cfg, c = _serve_config(**kw)
+    n = c["n_requests"]
+    out = dict(arrival_s=np.zeros(n), nk=np.zeros(n, np.int32), beta=np.zeros(n, np.float32),
+               n_chains=np.zeros(n, np.int32), shear_rate=np.zeros(n), bytes=np.zeros(n, np.int64))
+    _check(load_library().dsm_serve_plan(C.byref(cfg), n, _p(out["arrival_s"], C.c_double), _p(out["nk"], C.c_int32),
+                                         _p(out["beta"], C.c_float), _p(out["n_chains"], C.c_int32),
+                                         _p(out["shear_rate"], C.c_double), _p(out["bytes"], C.c_int64)))
+    return out
+
+
+def serve(device=0, **kw):
+    """Run a serving stream on the device; returns (report dict, per-request latency in s)."""
+    cfg, c = _serve_config(**kw)
+    rep = ServeReport()
+    lat = np.zeros(c["n_requests"])
+    _check(load_library().dsm_serve(device, C.byref(cfg), C.byref(rep), _p(lat, C.c_double)))
+    return rep.as_dict(), lat
 
 
 def run_main(job_id=0, savefile=None, loadfile=None, device=0, distr=False, mode="batched", corr_p=64, corr_m=8,

@@ -1,0 +1,94 @@
+"""Reference workloads of the DSM ensemble path, one JSON line each.
+
+    python -m gpu_dsm_amd.workloads c1 [--horizon T]      1000 chains, Nk=25, CD off, G(t)
+    python -m gpu_dsm_amd.workloads c4 [--horizon T]      16 sheared ensembles x 16384 chains, Nk=50
+    python -m gpu_dsm_amd.workloads c5 [--horizon T]      serving stream, 2000 requests at 20/s
+
+The headline workload (131,072 chains, Nk=100, CD on, G(t)) is bench.py at the repo
+root.  Every run starts from the equilibrium draw; all inputs are synthetic
+(Philox-keyed); times are wall clock around the device work.
+"""
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import engine as eng
+
+
+def c1(horizon=1.0e5, seed=1):
+    """1000 chains, Nk=25, beta=1, CD off, G(t) with p=16, m=2 (the small CLI case)."""
+    n, nk, p, m = 1000, 25, 16, 2
+    s = int(math.ceil(math.log(horizon / p) / math.log(m))) + 1
+    with eng.Engine(0) as e:
+        ens = e.add_ensemble(n, nk, 1.0, 0, seed=seed, sample="gt", delta=1.0, corr_p=p, corr_m=m, corr_levels=s)
+        e.init_equilibrium(ens)
+        z0 = e.state(ens)["Z"]
+        t0 = time.perf_counter()
+        st = e.advance(ens, horizon)
+        wall = time.perf_counter() - t0
+        lag, g = e.correlator(ens)
+    return dict(workload="c1: 1000 chains, Nk=25, beta=1, CD off, G(t) p=16 m=2", horizon_tau_k=horizon,
+                jumps=st["events"], seconds=wall, jumps_per_s=st["events"] / wall,
+                G0=float(g[0]), G0_from_initial_Z=float(np.mean(np.maximum(z0 - 2, 0))),
+                npoints=int(len(g)))
+
+
+def c4(horizon=200.0, seed=1):
+    """16 ensembles x 16384 chains, Nk=50, beta=1, CD on, shear rates 1e-5..1e-1 per
+    tau_K (log-spaced), all advanced in one batched pass per output window."""
+    n, nk, k = 16384, 50, 16
+    rates = [10.0 ** (-5.0 + 4.0 * q / (k - 1)) for q in range(k)]
+    with eng.Engine(0) as e:
+        ids = []
+        for q, gd in enumerate(rates):
+            ens = e.add_ensemble(n, nk, 1.0, 1, kappa=[0, gd, 0, 0, 0, 0, 0, 0, 0], seed=seed, salt=q,
+                                 sample="flow", delta=1.0, window=64)
+            e.init_equilibrium(ens)
+            ids.append(ens)
+        t0 = time.perf_counter()
+        st = e.advance_many(ids, horizon)
+        wall = time.perf_counter() - t0
+        out = []
+        for ens, gd in zip(ids, rates):
+            rows = e.flow_stress(ens)
+            half = rows[rows.shape[0] // 2:]
+            sxy = float(np.mean(half[:, 3] / half[:, 8]))
+            n1 = float(np.mean((half[:, 0] - half[:, 1]) / half[:, 8]))
+            out.append(dict(rate=gd, sigma_xy=sxy, N1=n1))
+    return dict(workload="c4: 16 x 16384 chains, Nk=50, beta=1, CD on, shear 1e-5..1e-1", horizon_tau_k=horizon,
+                jumps=st["events"], seconds=wall, jumps_per_s=st["events"] / wall,
+                note="stress averages over the second half of a short run (not steady state at low rates)",
+                per_rate=out)
+
+
+def c5(horizon=20.0, window=5.0, seed=1, n_requests=2000, arrival_rate=20.0):
+    """Serving stream: 2000 requests at 20/s, Nk 16..512, beta {0.5,1,2}, 1024..65536
+    chains, half sheared; `horizon` tau_K per request (scaled; see the note)."""
+    rep, lat = eng.serve(horizon=horizon, window=window, seed=seed, n_requests=n_requests,
+                         arrival_rate=arrival_rate)
+    plan = eng.serve_plan(horizon=horizon, seed=seed, n_requests=n_requests, arrival_rate=arrival_rate)
+    rep.update(workload="c5: serving stream, %d requests at %g/s" % (n_requests, arrival_rate),
+               horizon_tau_k=horizon, window_tau_k=window,
+               offered_chains=int(plan["n_chains"].sum()),
+               note="scaled horizon: the stated 1e5 tau_K per request is about 1.7e14 jumps in total")
+    return rep
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["c1", "c4", "c5"])
+    ap.add_argument("--horizon", type=float, default=None)
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    fn = dict(c1=c1, c4=c4, c5=c5)[args.which]
+    kw = dict(seed=args.seed)
+    if args.horizon is not None:
+        kw["horizon"] = args.horizon
+    print(json.dumps(fn(**kw)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

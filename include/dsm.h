@@ -168,6 +168,46 @@ int dsm_main(volatile const bool* run_flag, int job_ID, const char* savefile, co
 int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char* savefile, const char* loadfile,
                 int device_ID, bool distr, volatile int* progress_bar, const dsm_main_options* opt);
 
+/* Serving (no counterpart in the reference, whose driver runs one job per process):
+   a seeded stream of simulation requests with Poisson arrivals is admitted into one
+   engine while the requests' estimated device memory fits hbm_fraction of the free
+   HBM, every admitted request advances by `window` tau_K per scheduling round
+   (one dsm_advance_many pass), and a request completes when its horizon is reached
+   and its result (G(t), or flow stress rows when sheared) has been read back. */
+typedef struct {
+  int32_t n_requests;
+  double arrival_rate;       /* requests per second (wall clock) */
+  uint64_t seed;             /* request stream; also Philox key word 0 (salt = request index) */
+  int32_t nk_min, nk_max, nk_step;  /* Nk uniform over nk_min, nk_min+step, ..., <= nk_max (<= 1023) */
+  int32_t n_beta;            /* 1..4 */
+  float beta[4];             /* beta uniform over the first n_beta entries */
+  int32_t nch_min, nch_max;  /* chains per request, log-uniform */
+  double flow_fraction;      /* share of requests under shear (others: equilibrium G(t)) */
+  double rate_min, rate_max; /* shear rate kappa_xy, log-uniform, 1/tau_K */
+  double horizon;            /* tau_K simulated per request */
+  double window;             /* tau_K per scheduling round */
+  int32_t cd;                /* constraint dynamics */
+  double hbm_fraction;       /* share of the free device memory admission may fill */
+} dsm_serve_config;
+
+typedef struct {
+  int32_t completed;
+  int32_t max_active;        /* most requests resident at once */
+  int32_t rounds;
+  double makespan_s;         /* first arrival window start to last completion */
+  double p50_s, p99_s, mean_s, max_s;  /* arrival-to-result latency */
+  uint64_t jumps;
+  double jumps_per_s;        /* aggregate over the makespan */
+  int64_t peak_bytes;        /* largest admitted memory estimate */
+} dsm_serve_report;
+
+/* Runs the stream on one device; latency_s (optional) receives n_requests values. */
+int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_serve_report* report, double* latency_s);
+/* The request stream alone (host only): arrival times, Nk, beta, chains, shear rate
+   (0 = equilibrium) and the admission memory estimate of each request. */
+int dsm_serve_plan(const dsm_serve_config* cfg, int32_t max_requests, double* arrival_s, int32_t* nk, float* beta,
+                   int32_t* n_chains, double* shear_rate, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif
