@@ -214,6 +214,12 @@ struct dsm_engine {
   std::vector<float> tabs;
   float* d_tabs = nullptr;
   size_t d_tabs_cap = 0;
+  struct TabKey {
+    int nmax, cd;
+    float beta, c_create;
+    int off;
+  };
+  std::vector<TabKey> tab_keys;  // tables are shared by ensembles with equal parameters
   std::vector<int64_t> stage_off;  // per ensemble
   int64_t stage_used = 0;
   RangeAlloc chain_space, strand_space, stage_space;
@@ -228,6 +234,8 @@ struct dsm_engine {
   int d_corr_cnt_cap = 0;
   double* d_red = nullptr;
   int d_red_cap = 0;
+  double* d_rows = nullptr;  // flow rows of one window, all ensembles
+  int64_t d_rows_cap = 0;
   int* d_healthy = nullptr;
   hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> host_work;
@@ -253,31 +261,57 @@ int grow(T** p, int64_t n_old, int64_t n_new, hipStream_t st) {
   return DSM_OK;
 }
 
-int grow_chains(dsm_engine* e, int n_new, int64_t s_new) {
+// capacity after growth to `need`: half again the current capacity, so that a stream of
+// additions copies each element a bounded number of times, unless the extra would take
+// more than a quarter of the free device memory (then exactly `need`)
+int64_t grown_capacity(int64_t cur, int64_t need, size_t bytes_per) {
+  if (need <= cur) return cur;
+  int64_t cap = std::max<int64_t>(need, cur + cur / 2);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || (size_t)(cap - need) * bytes_per > free_b / 4) cap = need;
+  return cap;
+}
+
+constexpr size_t kChainBytes = 7 * 4 + 8 + 8 + 8 + 4 + 8 + 8 + 8;  // per-chain arrays below
+constexpr size_t kStrandBytes = 5 * 4;                             // Qx Qy Qz N w
+
+int grow_chains(dsm_engine* e, int n_need, int64_t s_need) {
   ChainArrays& c = e->ch;
-  const int n0 = e->nch;
-  const int64_t s0 = e->nstr;
   int rc = 0;
-  if ((rc = grow(&c.Z, n0, n_new, e->st)) || (rc = grow(&c.nk, n0, n_new, e->st)) ||
-      (rc = grow(&c.ens, n0, n_new, e->st)) || (rc = grow(&c.cap, n0, n_new, e->st)) ||
-      (rc = grow(&c.stall, n0, n_new, e->st)) || (rc = grow(&c.cid, n0, n_new, e->st)) ||
-      (rc = grow(&c.local, n0, n_new, e->st)) || (rc = grow(&c.base, n0, n_new, e->st)) ||
-      (rc = grow(&c.t, n0, n_new, e->st)) || (rc = grow(&c.ev, n0, n_new, e->st)) ||
-      (rc = grow(&c.dt_prev, n0, n_new, e->st)) || (rc = grow(&c.next_sample, n0, n_new, e->st)) ||
-      (rc = grow(&c.ev_stop, n0, n_new, e->st)) || (rc = grow(&c.zsum, n0, n_new, e->st)) ||
-      (rc = grow(&c.Qx, s0, s_new, e->st)) || (rc = grow(&c.Qy, s0, s_new, e->st)) ||
-      (rc = grow(&c.Qz, s0, s_new, e->st)) || (rc = grow(&c.N, s0, s_new, e->st)) ||
-      (rc = grow(&c.w, s0, s_new, e->st)))
-    return rc;
-  // bin 0 pads every ensemble to whole quads; one spare list past the last bin
-  const int need = n_new + 4 * ((int)e->ens.size() + 2);
+  if (n_need > e->nch) {
+    const int n0 = e->nch;
+    const int n_new = (int)std::min<int64_t>(grown_capacity(n0, n_need, kChainBytes), INT32_MAX / 2);
+    if ((rc = grow(&c.Z, n0, n_new, e->st)) || (rc = grow(&c.nk, n0, n_new, e->st)) ||
+        (rc = grow(&c.ens, n0, n_new, e->st)) || (rc = grow(&c.cap, n0, n_new, e->st)) ||
+        (rc = grow(&c.stall, n0, n_new, e->st)) || (rc = grow(&c.cid, n0, n_new, e->st)) ||
+        (rc = grow(&c.local, n0, n_new, e->st)) || (rc = grow(&c.base, n0, n_new, e->st)) ||
+        (rc = grow(&c.t, n0, n_new, e->st)) || (rc = grow(&c.ev, n0, n_new, e->st)) ||
+        (rc = grow(&c.dt_prev, n0, n_new, e->st)) || (rc = grow(&c.next_sample, n0, n_new, e->st)) ||
+        (rc = grow(&c.ev_stop, n0, n_new, e->st)) || (rc = grow(&c.zsum, n0, n_new, e->st)))
+      return rc;
+    e->nch = n_new;
+  }
+  if (s_need > e->nstr) {
+    const int64_t s0 = e->nstr;
+    const int64_t s_new = grown_capacity(s0, s_need, kStrandBytes);
+    if ((rc = grow(&c.Qx, s0, s_new, e->st)) || (rc = grow(&c.Qy, s0, s_new, e->st)) ||
+        (rc = grow(&c.Qz, s0, s_new, e->st)) || (rc = grow(&c.N, s0, s_new, e->st)) ||
+        (rc = grow(&c.w, s0, s_new, e->st)))
+      return rc;
+    e->nstr = s_new;
+  }
+  // bin 0 pads every live ensemble to whole quads; one spare list past the last bin
+  int live = 1;
+  for (const Ens& E : e->ens) live += E.live ? 1 : 0;
+  const int need = e->nch + 4 * (live + 2);
   if (need > e->d_work_cap) {
     if (e->d_work) HIPCHK(hipFree(e->d_work));
-    if ((rc = dev_alloc(&e->d_work, (size_t)(kNumBins + 1) * need))) return rc;
-    e->d_work_cap = need;
+    e->d_work = nullptr;
+    e->d_work_cap = 0;
+    const int cap = need + need / 8 + 1024;
+    if ((rc = dev_alloc(&e->d_work, (size_t)(kNumBins + 1) * cap))) return rc;
+    e->d_work_cap = cap;
   }
-  e->nch = n_new;
-  e->nstr = s_new;
   return DSM_OK;
 }
 
@@ -310,9 +344,10 @@ int upload_tabs(dsm_engine* e) {
 
 int ensure_stage(dsm_engine* e) {
   if (e->stage_used > e->d_stage_cap) {  // keeps the rows of live ensembles (last_samples)
-    int rc = grow(&e->d_stage, e->d_stage_cap, e->stage_used, e->st);
+    const int64_t cap = grown_capacity(e->d_stage_cap, e->stage_used, sizeof(float));
+    int rc = grow(&e->d_stage, e->d_stage_cap, cap, e->st);
     if (rc) return rc;
-    e->d_stage_cap = e->stage_used;
+    e->d_stage_cap = cap;
   }
   return DSM_OK;
 }
@@ -569,35 +604,63 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
     rc = kmc_pass(e, act, mode, false, run_flag, &rs);
     if (rc == DSM_ABORTED) { finish_stats(e, ids, before, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
     if (rc) return rc;
-    // consume the window's samples
-    for (int id : act) {
-      Ens& E = e->ens[id];
-      const int64_t rows = E.P.k_end - E.P.k0;
-      if (rows <= 0) continue;
-      E.last_k0 = E.P.k0;
-      E.last_rows = rows;
+    // consume the window's samples: every reduction is queued, then one synchronisation
+    std::vector<int> cnt;
+    std::vector<int64_t> cnt_off(act.size(), -1), row_off(act.size(), -1);
+    int64_t flow_doubles = 0;
+    for (size_t q = 0; q < act.size(); ++q) {
+      const Ens& E = e->ens[act[q]];
+      if (E.P.k_end <= E.P.k0) continue;
       if (E.P.sample_mode == kSampleEq) {
-        std::vector<int> cnt(3 * E.s);
-        for (int k = 0; k < E.s; ++k) {
-          cnt[k] = E.ins[k];
-          cnt[E.s + k] = E.macc[k];
-          cnt[2 * E.s + k] = (int)std::min<int64_t>(E.nins[k], E.p);
-        }
-        HIPCHK(hipMemcpyAsync(e->d_corr_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, e->st));
-        HIPCHK(launch_corr_update(e->d_stage + E.P.stage_off, E.dD, E.dA, E.dC, e->ch.stall, E.chain0, E.n, E.p, E.m,
-                                  E.s, (int)rows, e->d_corr_cnt, e->st));
-        HIPCHK(hipStreamSynchronize(e->st));
-        corr_advance_counters(E, rows);
+        cnt_off[q] = (int64_t)cnt.size();
+        for (int k = 0; k < E.s; ++k) cnt.push_back(E.ins[k]);
+        for (int k = 0; k < E.s; ++k) cnt.push_back(E.macc[k]);
+        for (int k = 0; k < E.s; ++k) cnt.push_back((int)std::min<int64_t>(E.nins[k], E.p));
       } else {
-        double* d_rows = nullptr;
-        if ((rc = dev_alloc(&d_rows, (size_t)rows * 9))) return rc;
-        HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, e->ch.stall, E.chain0, E.n, (int)rows, d_rows, e->st));
-        const size_t old = E.flow_rows.size();
-        E.flow_rows.resize(old + (size_t)rows * 9);
-        HIPCHK(hipMemcpyAsync(E.flow_rows.data() + old, d_rows, sizeof(double) * rows * 9, hipMemcpyDeviceToHost,
-                              e->st));
-        HIPCHK(hipStreamSynchronize(e->st));
-        HIPCHK(hipFree(d_rows));
+        row_off[q] = flow_doubles;
+        flow_doubles += (E.P.k_end - E.P.k0) * 9;
+      }
+    }
+    if ((int)cnt.size() > e->d_corr_cnt_cap) {
+      if (e->d_corr_cnt) HIPCHK(hipFree(e->d_corr_cnt));
+      if ((rc = dev_alloc(&e->d_corr_cnt, cnt.size()))) return rc;
+      e->d_corr_cnt_cap = (int)cnt.size();
+    }
+    if (flow_doubles > e->d_rows_cap) {
+      if (e->d_rows) HIPCHK(hipFree(e->d_rows));
+      if ((rc = dev_alloc(&e->d_rows, (size_t)flow_doubles))) return rc;
+      e->d_rows_cap = flow_doubles;
+    }
+    if (!cnt.empty())
+      HIPCHK(hipMemcpyAsync(e->d_corr_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, e->st));
+    for (size_t q = 0; q < act.size(); ++q) {
+      Ens& E = e->ens[act[q]];
+      const int rows = (int)(E.P.k_end - E.P.k0);
+      if (rows <= 0) continue;
+      if (E.P.sample_mode == kSampleEq)
+        HIPCHK(launch_corr_update(e->d_stage + E.P.stage_off, E.dD, E.dA, E.dC, e->ch.stall, E.chain0, E.n, E.p, E.m,
+                                  E.s, rows, e->d_corr_cnt + cnt_off[q], e->st));
+      else
+        HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, e->ch.stall, E.chain0, E.n, rows,
+                                  e->d_rows + row_off[q], e->st));
+    }
+    std::vector<double> flow_host((size_t)flow_doubles);
+    if (flow_doubles)
+      HIPCHK(hipMemcpyAsync(flow_host.data(), e->d_rows, sizeof(double) * flow_doubles, hipMemcpyDeviceToHost,
+                            e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    for (size_t q = 0; q < act.size(); ++q) {
+      Ens& E = e->ens[act[q]];
+      const int64_t rows = E.P.k_end - E.P.k0;
+      if (rows > 0) {
+        E.last_k0 = E.P.k0;
+        E.last_rows = rows;
+        if (E.P.sample_mode == kSampleEq) {
+          corr_advance_counters(E, rows);
+        } else {
+          const double* src = flow_host.data() + row_off[q];
+          E.flow_rows.insert(E.flow_rows.end(), src, src + rows * 9);
+        }
       }
       E.k_cur = E.P.k_end;
     }
@@ -674,7 +737,7 @@ int dsm_destroy(dsm_engine* e) {
   ChainArrays& c = e->ch;
   void* ptrs[] = {c.Z, c.nk, c.ens, c.cap, c.stall, c.cid, c.local, c.base, c.t, c.ev, c.dt_prev, c.next_sample, c.ev_stop, c.zsum,
                   c.Qx, c.Qy, c.Qz, c.N, c.w, e->d_ens, e->d_tabs, e->d_stage, e->d_work, e->d_cnt, e->d_counters,
-                  e->d_corr_cnt, e->d_red, e->d_healthy};
+                  e->d_corr_cnt, e->d_red, e->d_rows, e->d_healthy};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& ev : e->evs)
@@ -739,12 +802,18 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   P.k0 = 0;
   P.k_end = 0;
   // tables
-  std::vector<float> tb;
-  build_tables(tb, nmax, d->beta, P.cd, P.c_create);
-  P.tab_off = (int)e->tabs.size();
+  P.tab_off = -1;
+  for (const auto& k : e->tab_keys)
+    if (k.nmax == nmax && k.cd == P.cd && k.beta == d->beta && k.c_create == P.c_create) P.tab_off = k.off;
+  if (P.tab_off < 0) {
+    std::vector<float> tb;
+    build_tables(tb, nmax, d->beta, P.cd, P.c_create);
+    P.tab_off = (int)e->tabs.size();
+    e->tabs.insert(e->tabs.end(), tb.begin(), tb.end());
+    e->tab_keys.push_back({nmax, P.cd, d->beta, P.c_create, P.tab_off});
+    if ((rc = upload_tabs(e))) return rc;
+  }
   P.fr_off = P.tab_off + kNumTables * (nmax + 2);
-  e->tabs.insert(e->tabs.end(), tb.begin(), tb.end());
-  if ((rc = upload_tabs(e))) return rc;
   // sampling
   E.ncomp = d->sample_mode == DSM_SAMPLE_GT ? 3 : d->sample_mode == DSM_SAMPLE_FLOW ? 8 : 0;
   E.window = d->window > 0 ? d->window : kDefaultWindow;
@@ -762,9 +831,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   for (int i = 0; i < E.n; ++i) E.nstrand += cap_of(E.nk[i]);
   P.chain0 = E.chain0;
   E.strand0 = e->strand_space.take(E.nstrand);
-  if ((rc = grow_chains(e, (int)std::max<int64_t>(e->nch, e->chain_space.end),
-                        std::max<int64_t>(e->nstr, e->strand_space.end))))
-    return rc;
+  if ((rc = grow_chains(e, (int)e->chain_space.end, e->strand_space.end))) return rc;
   {
     const int id = (int)e->ens.size();
     std::vector<int> hnk(E.n), hens(E.n, id), hcap(E.n), hzero(E.n, 0), hloc(E.n);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -45,7 +46,7 @@ bool valid(const dsm_serve_config* c) {
          c->nk_max <= 1023 && c->nk_step > 0 && c->n_beta >= 1 && c->n_beta <= 4 && c->nch_min >= 1 &&
          c->nch_max >= c->nch_min && c->flow_fraction >= 0.0 && c->flow_fraction <= 1.0 &&
          (c->flow_fraction == 0.0 || (c->rate_min > 0.0 && c->rate_max >= c->rate_min)) && c->horizon > 0.0 &&
-         c->window > 0.0 && c->hbm_fraction > 0.0 && c->hbm_fraction <= 1.0;
+         c->window > 0.0 && c->hbm_fraction > 0.0 && c->hbm_fraction <= 1.0 && c->progress_s >= 0.0;
 }
 
 constexpr int kCorrP = 16, kCorrM = 2, kStageRows = 16;
@@ -134,8 +135,20 @@ extern "C" int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_ser
   int64_t used = 0;
   int completed = 0;
   std::vector<double> corr_lag, corr_g, flow_rows;
+  double next_note = cfg->progress_s;
+  auto note = [&](double tn) {
+    if (cfg->progress_s <= 0.0 || tn < next_note) return;
+    const size_t arrived = (size_t)(std::upper_bound(rq.begin(), rq.end(), tn,
+                                                     [](double x, const Req& r) { return x < r.arrival; }) -
+                                    rq.begin());
+    std::fprintf(stderr, "serve: %.1f s  arrived %zu  admitted %zu  completed %d  active %zu  rounds %d  jumps %.3g\n",
+                 tn, arrived, next, completed, active.size(), report->rounds, (double)report->jumps);
+    std::fflush(stderr);
+    next_note = tn + cfg->progress_s;
+  };
   while (completed < (int)rq.size()) {
     const double tn = now();
+    note(tn);
     // admission: arrived requests, in order, while their memory fits
     while (next < rq.size() && rq[next].arrival <= tn) {
       Req& r = rq[next];
@@ -159,7 +172,10 @@ extern "C" int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_ser
         d.corr_m = kCorrM;
         d.corr_levels = corr_levels(cfg->horizon);
       }
+      const double ta = now();
       if ((rc = dsm_add_ensemble(e, &d, &r.ens)) || (rc = dsm_init_equilibrium(e, r.ens))) break;
+      report->admit_s += now() - ta;
+      note(now());
       used += r.bytes;
       report->peak_bytes = std::max<int64_t>(report->peak_bytes, used);
       active.push_back((int)next);
@@ -170,7 +186,8 @@ extern "C" int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_ser
     if (active.empty()) {
       if (next < rq.size()) {
         const double wait = rq[next].arrival - now();
-        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+        const double nap = cfg->progress_s > 0.0 ? std::min(wait, cfg->progress_s) : wait;
+        if (nap > 0) std::this_thread::sleep_for(std::chrono::duration<double>(nap));
       }
       continue;
     }
@@ -182,8 +199,11 @@ extern "C" int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_ser
       te.push_back(std::min(cfg->horizon, rq[i].t + cfg->window));
     }
     dsm_run_stats st{};
+    const double tr = now();
     if ((rc = dsm_advance_many(e, ids.data(), te.data(), (int32_t)ids.size(), DSM_MODE_BATCHED, nullptr, &st)))
       break;
+    const double tf = now();
+    report->advance_s += tf - tr;
     report->jumps += st.events;
     report->rounds++;
     std::vector<int> still;
@@ -209,6 +229,7 @@ extern "C" int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_ser
       r.done = now();
       ++completed;
     }
+    report->fetch_s += now() - tf;
     if (rc) break;
     active.swap(still);
   }

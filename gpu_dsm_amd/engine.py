@@ -51,14 +51,16 @@ class ServeConfig(C.Structure):
                 ("nk_min", C.c_int32), ("nk_max", C.c_int32), ("nk_step", C.c_int32), ("n_beta", C.c_int32),
                 ("beta", C.c_float * 4), ("nch_min", C.c_int32), ("nch_max", C.c_int32),
                 ("flow_fraction", C.c_double), ("rate_min", C.c_double), ("rate_max", C.c_double),
-                ("horizon", C.c_double), ("window", C.c_double), ("cd", C.c_int32), ("hbm_fraction", C.c_double)]
+                ("horizon", C.c_double), ("window", C.c_double), ("cd", C.c_int32), ("hbm_fraction", C.c_double),
+                ("progress_s", C.c_double)]
 
 
 class ServeReport(C.Structure):
     _fields_ = [("completed", C.c_int32), ("max_active", C.c_int32), ("rounds", C.c_int32),
                 ("makespan_s", C.c_double), ("p50_s", C.c_double), ("p99_s", C.c_double), ("mean_s", C.c_double),
                 ("max_s", C.c_double), ("jumps", C.c_uint64), ("jumps_per_s", C.c_double),
-                ("peak_bytes", C.c_int64)]
+                ("peak_bytes", C.c_int64), ("admit_s", C.c_double), ("advance_s", C.c_double),
+                ("fetch_s", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -69,7 +71,8 @@ class ServeReport(C.Structure):
 # (about 1.7e14 jumps in total; callers pass a shorter horizon for timed runs)
 SERVE_DEFAULTS = dict(n_requests=2000, arrival_rate=20.0, seed=1, nk_min=16, nk_max=512, nk_step=16,
                       beta=(0.5, 1.0, 2.0), nch_min=1024, nch_max=65536, flow_fraction=0.5, rate_min=1e-5,
-                      rate_max=1e-2, horizon=1.0e5, window=10.0, cd=1, hbm_fraction=0.8)
+                      rate_max=1e-2, horizon=1.0e5, window=10.0, cd=1, hbm_fraction=0.8,
+                      progress_s=0.0)
 
 
 class MainOptions(C.Structure):

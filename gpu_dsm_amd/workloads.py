@@ -1,6 +1,7 @@
 """Reference workloads of the DSM ensemble path, one JSON line each.
 
     python -m gpu_dsm_amd.workloads c1 [--horizon T]      1000 chains, Nk=25, CD off, G(t)
+    python -m gpu_dsm_amd.workloads c3 [--horizon T]      262,144 chains, log-normal Nk, CD on, G(t)
     python -m gpu_dsm_amd.workloads c4 [--horizon T]      16 sheared ensembles x 16384 chains, Nk=50
     python -m gpu_dsm_amd.workloads c5 [--horizon T]      serving stream, 2000 requests at 20/s
 
@@ -36,6 +37,39 @@ def c1(horizon=1.0e5, seed=1):
                 npoints=int(len(g)))
 
 
+def c3_lengths(n=262144, seed=1, nk_lo=10, nk_hi=1023):
+    """Chain lengths of the polydisperse melt: Nk = round(LogNormal(mu, sigma)) with
+    number-average 200 and Mw/Mn = 1.5 (sigma^2 = ln 1.5, mu = ln 200 - sigma^2/2),
+    clipped to [nk_lo, nk_hi]."""
+    s2 = math.log(1.5)
+    rng = np.random.default_rng(seed)
+    nk = np.rint(rng.lognormal(math.log(200.0) - s2 / 2, math.sqrt(s2), n))
+    return np.clip(nk, nk_lo, nk_hi).astype(np.int32)
+
+
+def c3(horizon=100.0, seed=1, n=262144):
+    """Polydisperse melt: 262,144 chains of log-normal length (Mn 200, PDI 1.5), beta=1,
+    CD on, G(t) with p=16, m=2, all chains in one batched pass per output window."""
+    nk = c3_lengths(n, seed)
+    p, m = 16, 2
+    s = int(math.ceil(math.log(horizon / p) / math.log(m))) + 1
+    with eng.Engine(0) as e:
+        ens = e.add_ensemble(n, int(nk.max()), 1.0, 1, seed=seed, sample="gt", delta=1.0, corr_p=p, corr_m=m,
+                             corr_levels=s, nk_per_chain=nk)
+        e.init_equilibrium(ens)
+        z0 = e.state(ens)["Z"]
+        t0 = time.perf_counter()
+        st = e.advance(ens, horizon)
+        wall = time.perf_counter() - t0
+        z1 = e.state(ens)["Z"]
+        lag, g = e.correlator(ens)
+    return dict(workload="c3: %d chains, Nk log-normal Mn 200 PDI 1.5 clipped to [10, %d], beta=1, CD on, G(t)"
+                % (n, int(nk.max())), horizon_tau_k=horizon, jumps=st["events"], seconds=wall,
+                jumps_per_s=st["events"] / wall, kmc_ms=st["kmc_ms"], first_kmc_ms=st["first_kmc_ms"],
+                mean_Z_start=float(z0.mean()), mean_Z_end=float(z1.mean()), max_Z=int(z1.max()),
+                chains_Z_over_64=int((z1 > 64).sum()), G0=float(g[0]))
+
+
 def c4(horizon=200.0, seed=1):
     """16 ensembles x 16384 chains, Nk=50, beta=1, CD on, shear rates 1e-5..1e-1 per
     tau_K (log-spaced), all advanced in one batched pass per output window."""
@@ -60,7 +94,8 @@ def c4(horizon=200.0, seed=1):
             out.append(dict(rate=gd, sigma_xy=sxy, N1=n1))
     return dict(workload="c4: 16 x 16384 chains, Nk=50, beta=1, CD on, shear 1e-5..1e-1", horizon_tau_k=horizon,
                 jumps=st["events"], seconds=wall, jumps_per_s=st["events"] / wall,
-                note="stress averages over the second half of a short run (not steady state at low rates)",
+                note="stress averages over the second half of a short run (not steady state at low rates); "
+                     "sign convention of the reference's output: sigma = -sum 3 Q Q / N per chain",
                 per_rate=out)
 
 
@@ -68,7 +103,7 @@ def c5(horizon=20.0, window=5.0, seed=1, n_requests=2000, arrival_rate=20.0):
     """Serving stream: 2000 requests at 20/s, Nk 16..512, beta {0.5,1,2}, 1024..65536
     chains, half sheared; `horizon` tau_K per request (scaled; see the note)."""
     rep, lat = eng.serve(horizon=horizon, window=window, seed=seed, n_requests=n_requests,
-                         arrival_rate=arrival_rate)
+                         arrival_rate=arrival_rate, progress_s=10.0)
     plan = eng.serve_plan(horizon=horizon, seed=seed, n_requests=n_requests, arrival_rate=arrival_rate)
     rep.update(workload="c5: serving stream, %d requests at %g/s" % (n_requests, arrival_rate),
                horizon_tau_k=horizon, window_tau_k=window,
@@ -79,14 +114,17 @@ def c5(horizon=20.0, window=5.0, seed=1, n_requests=2000, arrival_rate=20.0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["c1", "c4", "c5"])
+    ap.add_argument("which", choices=["c1", "c3", "c4", "c5"])
     ap.add_argument("--horizon", type=float, default=None)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--requests", type=int, default=None, help="c5: number of requests")
     args = ap.parse_args()
-    fn = dict(c1=c1, c4=c4, c5=c5)[args.which]
+    fn = dict(c1=c1, c3=c3, c4=c4, c5=c5)[args.which]
     kw = dict(seed=args.seed)
     if args.horizon is not None:
         kw["horizon"] = args.horizon
+    if args.requests is not None and args.which == "c5":
+        kw["n_requests"] = args.requests
     print(json.dumps(fn(**kw)), flush=True)
 
 

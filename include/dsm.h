@@ -188,6 +188,7 @@ typedef struct {
   double window;             /* tau_K per scheduling round */
   int32_t cd;                /* constraint dynamics */
   double hbm_fraction;       /* share of the free device memory admission may fill */
+  double progress_s;         /* > 0: one status line on stderr every progress_s seconds */
 } dsm_serve_config;
 
 typedef struct {
@@ -199,6 +200,8 @@ typedef struct {
   uint64_t jumps;
   double jumps_per_s;        /* aggregate over the makespan */
   int64_t peak_bytes;        /* largest admitted memory estimate */
+  double admit_s, advance_s, fetch_s;  /* host wall time in admission (allocation and
+                                          equilibrium draw), KMC rounds, result retrieval */
 } dsm_serve_report;
 
 /* Runs the stream on one device; latency_s (optional) receives n_requests values. */

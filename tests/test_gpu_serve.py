@@ -37,3 +37,70 @@ def test_removed_ensembles_are_reused(engine, oracle):
         engine.remove_ensemble(ens)
         free.append(g.mem_info(0)[0])
     assert free[-1] >= free[1] - (64 << 20)
+
+
+MIX = [  # (chains, Nk, beta, shear rate, admitted before round)
+    (300, 40, 1.0, 0.0, 0),
+    (200, 64, 0.5, 1e-2, 0),
+    (100, 200, 2.0, 0.0, 1),     # Z crosses 64: row form hands chains to the wave form
+    (150, 500, 0.5, 3e-3, 1),    # Z ~ 330: wave form, two 256-position groups
+    (250, 30, 1.0, 0.0, 3),      # admitted into the ranges freed by the first two
+]
+
+
+def _add(e, q, n, nk, beta, rate, horizon):
+    if rate > 0.0:
+        return e.add_ensemble(n, nk, beta, 1, kappa=[0, rate, 0, 0, 0, 0, 0, 0, 0], seed=77, salt=q, sample="flow",
+                              delta=1.0, window=16)
+    return e.add_ensemble(n, nk, beta, 1, seed=77, salt=q, sample="gt", delta=1.0, corr_p=16, corr_m=2,
+                          corr_levels=2, window=16)
+
+
+def _result(e, ens, rate):
+    st = e.state(ens)
+    out = dict(Z=st["Z"], N=st["N"], Q=st["Q"], w=st["w"], t=st["t"], ev=st["ev"])
+    if rate > 0.0:
+        out["rows"] = e.flow_stress(ens)
+    else:
+        out["lag"], out["G"] = e.correlator(ens)
+    return out
+
+
+def test_results_same_alone_and_in_the_mix():
+    """Continuous batching changes no result: requests admitted at different rounds,
+    advanced together (one batched pass per 5 tau_K window over every active request)
+    and removed when done give the same chain state, G(t) and flow stress rows as each
+    request run alone in a fresh engine.  Tolerance: none (bit-identical), since every
+    chain's random stream is keyed by (seed, request, chain) and the reductions sum
+    in a fixed order relative to the request's first chain."""
+    import gpu_dsm_amd as g
+    horizon, window = 15.0, 5.0
+    mixed, active, t = {}, {}, {}
+    with g.Engine(0) as e:
+        rnd = 0
+        while len(mixed) < len(MIX):
+            for q, (n, nk, beta, rate, when) in enumerate(MIX):
+                if when == rnd:
+                    active[q] = _add(e, q, n, nk, beta, rate, horizon)
+                    e.init_equilibrium(active[q])
+                    t[q] = 0.0
+            qs = sorted(active)
+            e.advance_many([active[q] for q in qs], [min(horizon, t[q] + window) for q in qs])
+            for q in qs:
+                t[q] = min(horizon, t[q] + window)
+                if t[q] >= horizon:
+                    mixed[q] = _result(e, active[q], MIX[q][3])
+                    e.remove_ensemble(active.pop(q))
+            rnd += 1
+            assert rnd < 20
+    for q, (n, nk, beta, rate, _) in enumerate(MIX):
+        with g.Engine(0) as e:
+            ens = _add(e, q, n, nk, beta, rate, horizon)
+            e.init_equilibrium(ens)
+            for k in range(1, 4):
+                e.advance(ens, k * window)
+            alone = _result(e, ens, rate)
+        for key, v in alone.items():
+            assert np.array_equal(v, mixed[q][key]), "request %d: %s differs alone vs in the mix" % (q, key)
+        assert int(alone["Z"].max()) > 0
+    assert int(mixed[2]["Z"].max()) > 64 or int(mixed[3]["Z"].max()) > 256
