@@ -233,10 +233,6 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
     std::cout << "PD_flag=1 (GEX polydisperse CD background from polydisp.dat) is not supported by this build\n";
     return DSM_ERR_ARG;
   }
-  if (!flow && in.equilibrium_type == 2) {
-    std::cout << "MSD(t) (equilibrium_type 2) is not supported by this build\n";
-    return DSM_ERR_ARG;
-  }
   dsm_engine* e = nullptr;
   int rc = dsm_create(device_ID, &e);
   if (rc) {
@@ -253,7 +249,9 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
   d.seed = (uint32_t)job_ID;
   d.salt = 0;
   d.delta = flow ? in.t_step_size : (double)(int)in.t_step_size;  // G(t) uses an integer interval (common/ensemble.cu:289)
-  d.sample_mode = flow ? DSM_SAMPLE_FLOW : (in.equilibrium_type == 1 ? DSM_SAMPLE_GT : DSM_SAMPLE_NONE);
+  d.sample_mode = flow ? DSM_SAMPLE_FLOW
+                       : in.equilibrium_type == 1 ? DSM_SAMPLE_GT
+                       : in.equilibrium_type == 2 ? DSM_SAMPLE_MSD : DSM_SAMPLE_NONE;
   d.corr_p = o.corr_p;
   d.corr_m = o.corr_m;
   d.corr_levels = s;
@@ -332,8 +330,10 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
     }
     t_ref = (double)n_out * in.t_step_size;
     std::cout << "time evolution done.\n";
-  } else if (in.equilibrium_type == 1) {
-    std::cout << "G(t) calc...\n";
+  } else if (in.equilibrium_type == 1 || in.equilibrium_type == 2) {
+    // common/main_cuda.cu:142-155: G(t) (correlator type 0) or MSD(t) (type 1)
+    const bool msd = in.equilibrium_type == 2;
+    std::cout << (msd ? "MSD(t) calc...\n" : "G(t) calc...\n");
     std::cout.flush();
     const double T = in.simulation_time;
     const double step = std::max(1.0, d.delta) * (o.window > 0 ? o.window : 256) * 16;
@@ -357,7 +357,7 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
     // common/ensemble.cu:324-335: np - p + floor(T / (delta * m^(s-1))) points
     const int actual_np = (int)(np - o.corr_p +
                                 std::floor(T / ((float)(int)in.t_step_size * std::pow((float)o.corr_m, (float)(s - 1)))));
-    std::ofstream gf(filename_ID(dir, "G", job_ID));
+    std::ofstream gf(filename_ID(dir, msd ? "MSD" : "G", job_ID));
     std::cout << "\n";
     for (int j = 0; j < actual_np && j < np; ++j) {
       const float tj = (float)(int64_t)lag[j];

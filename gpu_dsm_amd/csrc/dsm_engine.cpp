@@ -172,6 +172,9 @@ int dev_alloc(T** p, size_t n) {
   return DSM_OK;
 }
 
+// G(t) and MSD(t) both sample the equilibrium state every delta into a multi-tau correlator
+inline bool eq_sampling(int mode) { return mode == kSampleEq || mode == kSampleMsd; }
+
 struct Ens {
   bool live = false;
   dsm_ensemble_desc d{};
@@ -272,7 +275,7 @@ int64_t grown_capacity(int64_t cur, int64_t need, size_t bytes_per) {
   return cap;
 }
 
-constexpr size_t kChainBytes = 7 * 4 + 8 + 8 + 8 + 4 + 8 + 8 + 8;  // per-chain arrays below
+constexpr size_t kChainBytes = 7 * 4 + 8 + 8 + 8 + 4 + 8 + 8 + 8 + 12;  // per-chain arrays below
 constexpr size_t kStrandBytes = 5 * 4;                             // Qx Qy Qz N w
 
 int grow_chains(dsm_engine* e, int n_need, int64_t s_need) {
@@ -287,7 +290,8 @@ int grow_chains(dsm_engine* e, int n_need, int64_t s_need) {
         (rc = grow(&c.local, n0, n_new, e->st)) || (rc = grow(&c.base, n0, n_new, e->st)) ||
         (rc = grow(&c.t, n0, n_new, e->st)) || (rc = grow(&c.ev, n0, n_new, e->st)) ||
         (rc = grow(&c.dt_prev, n0, n_new, e->st)) || (rc = grow(&c.next_sample, n0, n_new, e->st)) ||
-        (rc = grow(&c.ev_stop, n0, n_new, e->st)) || (rc = grow(&c.zsum, n0, n_new, e->st)))
+        (rc = grow(&c.ev_stop, n0, n_new, e->st)) || (rc = grow(&c.zsum, n0, n_new, e->st)) ||
+        (rc = grow(&c.R1, 3 * (int64_t)n0, 3 * (int64_t)n_new, e->st)))
       return rc;
     e->nch = n_new;
   }
@@ -411,8 +415,8 @@ struct Timer {
   }
 };
 
-int device_blocks(dsm_engine* e, int geom, bool plain, int lds_floats) {
-  int nb = kmc_max_blocks_per_cu(geom, plain, lds_floats);
+int device_blocks(dsm_engine* e, int geom, bool plain, int lds_floats, bool msd) {
+  int nb = kmc_max_blocks_per_cu(geom, plain, lds_floats, msd);
   if (nb <= 0) nb = 1;
   return nb * e->n_cu;
 }
@@ -448,8 +452,9 @@ struct Pass {
 };
 
 int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pass& ps) {
-  int n0 = 0, bin_max = 0, tab_max = 0;
+  int n0 = 0, bin_max = 0, tab_max = 0, msd = 0;
   for (int id : ids) {
+    msd |= e->ens[id].P.sample_mode == kSampleMsd ? 1 : 0;
     n0 += (e->ens[id].n + 3) & ~3;
     bin_max = std::max(bin_max, e->ens[id].bin_max);
     tab_max = std::max(tab_max, 9 * (e->ens[id].nmax + 2) + 4 * e->ens[id].nmax + 4);
@@ -467,6 +472,7 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   ps.a.use_ev_stop = use_ev_stop ? 1 : 0;
   // rate tables go to LDS (one copy per wave) when they fit the per-wave budget
   ps.a.lds_floats = tab_max <= kLdsTableFloats ? tab_max : 0;
+  ps.a.msd = msd;
   return DSM_OK;
 }
 
@@ -483,7 +489,7 @@ int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs
     a.queue = cnt + (kNumBins + 1) + b;
     a.overflow = e->d_work + (size_t)(b + 1) * e->d_work_cap;
     a.n_overflow = cnt + b + 1;
-    const int grid = device_blocks(e, b, plain, a.lds_floats);
+    const int grid = device_blocks(e, b, plain, a.lds_floats, a.msd != 0);
     if (timed) HIPCHK(hipEventRecord(e->evs[0], e->st));
     HIPCHK(launch_kmc(a, b, plain, grid, e->st));
     rs->kmc_launches++;
@@ -556,7 +562,7 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
   for (size_t q = 0; q < ids.size(); ++q) {
     Ens& E = e->ens[ids[q]];
     const double te = t_end[q];
-    if (E.P.sample_mode == kSampleEq) {
+    if (eq_sampling(E.P.sample_mode)) {
       k_target[q] = (int64_t)std::ceil(te / E.P.delta - 1e-12);
       if (E.s == 0) {
         int s = E.d.corr_levels;
@@ -589,7 +595,7 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
       const int64_t k_end = std::min(E.k_cur + E.window, k_target[q]);
       E.P.k0 = E.k_cur;
       E.P.k_end = k_end;
-      if (E.P.sample_mode == kSampleEq) E.P.t_end = std::min((double)k_end * E.P.delta, t_end[q]);
+      if (eq_sampling(E.P.sample_mode)) E.P.t_end = std::min((double)k_end * E.P.delta, t_end[q]);
       else E.P.t_end = (double)(k_end - 1) * E.P.delta;  // flow output k_end-1 is at this time
       if (k_end == k_target[q]) reached[q] = 1;
       act.push_back(ids[q]);
@@ -611,7 +617,7 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
     for (size_t q = 0; q < act.size(); ++q) {
       const Ens& E = e->ens[act[q]];
       if (E.P.k_end <= E.P.k0) continue;
-      if (E.P.sample_mode == kSampleEq) {
+      if (eq_sampling(E.P.sample_mode)) {
         cnt_off[q] = (int64_t)cnt.size();
         for (int k = 0; k < E.s; ++k) cnt.push_back(E.ins[k]);
         for (int k = 0; k < E.s; ++k) cnt.push_back(E.macc[k]);
@@ -637,9 +643,10 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
       Ens& E = e->ens[act[q]];
       const int rows = (int)(E.P.k_end - E.P.k0);
       if (rows <= 0) continue;
-      if (E.P.sample_mode == kSampleEq)
+      if (eq_sampling(E.P.sample_mode))
         HIPCHK(launch_corr_update(e->d_stage + E.P.stage_off, E.dD, E.dA, E.dC, e->ch.stall, E.chain0, E.n, E.p, E.m,
-                                  E.s, rows, e->d_corr_cnt + cnt_off[q], e->st));
+                                  E.s, rows, e->d_corr_cnt + cnt_off[q], E.P.sample_mode == kSampleMsd ? 1 : 0,
+                                  e->st));
       else
         HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, e->ch.stall, E.chain0, E.n, rows,
                                   e->d_rows + row_off[q], e->st));
@@ -655,7 +662,7 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
       if (rows > 0) {
         E.last_k0 = E.P.k0;
         E.last_rows = rows;
-        if (E.P.sample_mode == kSampleEq) {
+        if (eq_sampling(E.P.sample_mode)) {
           corr_advance_counters(E, rows);
         } else {
           const double* src = flow_host.data() + row_off[q];
@@ -735,7 +742,7 @@ int dsm_destroy(dsm_engine* e) {
     if (E.dC) (void)hipFree(E.dC);
   }
   ChainArrays& c = e->ch;
-  void* ptrs[] = {c.Z, c.nk, c.ens, c.cap, c.stall, c.cid, c.local, c.base, c.t, c.ev, c.dt_prev, c.next_sample, c.ev_stop, c.zsum,
+  void* ptrs[] = {c.Z, c.nk, c.ens, c.cap, c.stall, c.cid, c.local, c.base, c.t, c.ev, c.dt_prev, c.next_sample, c.ev_stop, c.zsum, c.R1,
                   c.Qx, c.Qy, c.Qz, c.N, c.w, e->d_ens, e->d_tabs, e->d_stage, e->d_work, e->d_cnt, e->d_counters,
                   e->d_corr_cnt, e->d_red, e->d_rows, e->d_healthy};
   for (void* p : ptrs)
@@ -751,6 +758,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   if (!e || !d) return fail(DSM_ERR_ARG, "null argument");
   if (d->n_chains <= 0) return fail(DSM_ERR_ARG, "n_chains must be positive");
   if (!(d->beta > 0.0f)) return fail(DSM_ERR_ARG, "beta must be positive");
+  if (d->sample_mode < DSM_SAMPLE_NONE || d->sample_mode > DSM_SAMPLE_MSD) return fail(DSM_ERR_ARG, "bad sample_mode");
   if ((d->sample_mode != DSM_SAMPLE_NONE) && !(d->delta > 0.0)) return fail(DSM_ERR_ARG, "delta must be positive");
   int rc = set_device(e);
   if (rc) return rc;
@@ -815,12 +823,13 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   }
   P.fr_off = P.tab_off + kNumTables * (nmax + 2);
   // sampling
-  E.ncomp = d->sample_mode == DSM_SAMPLE_GT ? 3 : d->sample_mode == DSM_SAMPLE_FLOW ? 8 : 0;
+  E.ncomp = (d->sample_mode == DSM_SAMPLE_GT || d->sample_mode == DSM_SAMPLE_MSD) ? 3
+            : d->sample_mode == DSM_SAMPLE_FLOW ? 8 : 0;
   E.window = d->window > 0 ? d->window : kDefaultWindow;
   E.k_cur = d->sample_mode == DSM_SAMPLE_FLOW ? 1 : 0;
   E.p = d->corr_p > 0 ? d->corr_p : 64;
   E.m = d->corr_m > 0 ? d->corr_m : 8;
-  if (E.p % E.m != 0 && d->sample_mode == DSM_SAMPLE_GT) return fail(DSM_ERR_ARG, "corr_p must be a multiple of corr_m");
+  if (E.p % E.m != 0 && eq_sampling(d->sample_mode)) return fail(DSM_ERR_ARG, "corr_p must be a multiple of corr_m");
   E.stage_floats = (int64_t)E.window * E.n * E.ncomp;
   P.stage_off = e->stage_space.take(E.stage_floats);
   e->stage_used = std::max(e->stage_used, e->stage_space.end);
@@ -854,6 +863,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
     HIPCHK(hipMemset(e->ch.t + c0, 0, 8 * E.n));
     HIPCHK(hipMemset(e->ch.ev + c0, 0, 8 * E.n));
     HIPCHK(hipMemset(e->ch.zsum + c0, 0, 8 * E.n));
+    HIPCHK(hipMemset(e->ch.R1 + 3 * (int64_t)c0, 0, 12 * (size_t)E.n));
     HIPCHK(hipMemset(e->ch.dt_prev + c0, 0, 4 * E.n));
     std::vector<int64_t> hns(E.n, E.k_cur);
     HIPCHK(hipMemcpy(e->ch.next_sample + c0, hns.data(), 8 * E.n, hipMemcpyHostToDevice));
@@ -887,6 +897,7 @@ int dsm_init_equilibrium(dsm_engine* e, int32_t ens) {
   if (rc) return rc;
   HIPCHK(launch_iota(e->d_work, E->chain0, E->n, e->st));
   HIPCHK(launch_init_eq(e->ch, e->d_ens, e->d_work, E->n, e->st));
+  HIPCHK(hipMemsetAsync(e->ch.R1 + 3 * (int64_t)E->chain0, 0, 12 * (size_t)E->n, e->st));  // common/chain.cu:207
   HIPCHK(hipStreamSynchronize(e->st));
   E->k_cur = E->P.sample_mode == kSampleFlow ? 1 : 0;
   E->flow_rows.clear();
@@ -897,6 +908,16 @@ int dsm_state_cap(dsm_engine* e, int32_t ens, int32_t* cap) {
   Ens* E = get_ens(e, ens);
   if (!E || !cap) return fail(DSM_ERR_ARG, "bad ensemble or pointer");
   *cap = E->nmax;
+  return DSM_OK;
+}
+
+int dsm_get_r1(dsm_engine* e, int32_t ens, float* r1) {
+  Ens* E = get_ens(e, ens);
+  if (!E || !r1) return fail(DSM_ERR_ARG, "bad ensemble or pointer");
+  int rc = set_device(e);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(e->st));
+  HIPCHK(hipMemcpy(r1, e->ch.R1 + 3 * (int64_t)E->chain0, 12 * (size_t)E->n, hipMemcpyDeviceToHost));
   return DSM_OK;
 }
 
@@ -1064,7 +1085,8 @@ int dsm_flow_stress(dsm_engine* e, int32_t ens, int64_t first_row, int64_t max_r
 int dsm_correlator(dsm_engine* e, int32_t ens, int32_t max_pts, double* lag, double* g, int32_t* npts) {
   Ens* E = get_ens(e, ens);
   if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
-  if (E->P.sample_mode != kSampleEq || !E->dC) return fail(DSM_ERR_STATE, "ensemble %d has no G(t) correlator", ens);
+  if (!eq_sampling(E->P.sample_mode) || !E->dC)
+    return fail(DSM_ERR_STATE, "ensemble %d has no G(t) or MSD(t) correlator", ens);
   int rc = set_device(e);
   if (rc) return rc;
   const int s = E->s, p = E->p, m = E->m;
@@ -1082,7 +1104,9 @@ int dsm_correlator(dsm_engine* e, int32_t ens, int32_t max_pts, double* lag, dou
       const double npair = (double)std::max<int64_t>(0, E->nins[k] - j);
       if (im < max_pts) {
         if (lag) lag[im] = (double)j * mk;
-        if (g) g[im] = (npair > 0 && healthy > 0) ? sums[k * p + j] / npair / 3.0 / healthy : 0.0;
+        // G: trace / 3 (common/correlator.cu:176); MSD: sum of the components (:177)
+        const double norm = E->P.sample_mode == kSampleMsd ? 1.0 : 3.0;
+        if (g) g[im] = (npair > 0 && healthy > 0) ? sums[k * p + j] / npair / norm / healthy : 0.0;
       }
       ++im;
     }
@@ -1151,6 +1175,9 @@ static int probe_run(int32_t device_id, const void* in, size_t in_bytes, void* o
 static hipError_t l_scan(const void* i, void* o, int n, int form, hipStream_t s) {
   return launch_scan_probe((const float*)i, (float*)o, n, form, s);
 }
+static hipError_t l_com(const void* i, void* o, int n, int g, hipStream_t s) {
+  return launch_com_probe((const float*)i, (float*)o, n, g, s);
+}
 static hipError_t l_math(const void* i, void* o, int n, int w, hipStream_t s) {
   return launch_math_probe((const float*)i, (float*)o, n, w, s);
 }
@@ -1163,6 +1190,11 @@ int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, in
     return fail(DSM_ERR_ARG, "bad probe arguments");
   const int nin = form == 1 ? n : 4 * n, nout = form == 1 ? n + 1 : 4 * (n + 1);
   return probe_run(device_id, in, 4 * (size_t)nin, out, 4 * (size_t)nout, l_scan, n, form);
+}
+int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom) {
+  if (!in || !out || geom < 0 || geom >= kNumGeoms || n < 1 || n > kmc_geom_capacity(geom))
+    return fail(DSM_ERR_ARG, "bad probe arguments");
+  return probe_run(device_id, in, 4 * (size_t)(4 * n + 1), out, 4 * 3, l_com, n, geom);
 }
 int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which) {
   if (n <= 0 || !in || !out || which < 0 || which > 3) return fail(DSM_ERR_ARG, "bad probe arguments");

@@ -158,6 +158,7 @@ struct CorrArgs {
   const int* stall;     // global stall flags
   int chain0, nch, p, m, s, nrows;
   const int* cnt;       // [3*s]: ins, macc, nins (clamped to p) at window start
+  int type;             // 0: stress (block averages, products); 1: MSD (first value, squared differences)
 };
 
 __global__ void __launch_bounds__(256) corr_update_kernel(CorrArgs a) {
@@ -179,12 +180,15 @@ __global__ void __launch_bounds__(256) corr_update_kernel(CorrArgs a) {
       D[(int64_t)(in * 3 + 1) * nch] = vy;
       D[(int64_t)(in * 3 + 2) * nch] = vz;
       double* A = a.A + (int64_t)(lvl * 3) * nch + c;
-      A[0] += vx; A[nch] += vy; A[2 * nch] += vz;
+      // MSD carries the first value of each block up unaveraged
+      // (common/eq_ensemble_kernel.cu:181-189)
+      if (a.type == 0 || macc[lvl] == 0) { A[0] += vx; A[nch] += vy; A[2 * nch] += vz; }
       macc[lvl]++;
       const bool push = macc[lvl] == m;
       double ux = 0.0, uy = 0.0, uz = 0.0;
       if (push) {
-        ux = A[0] / m; uy = A[nch] / m; uz = A[2 * nch] / m;
+        if (a.type == 0) { ux = A[0] / m; uy = A[nch] / m; uz = A[2 * nch] / m; }
+        else { ux = A[0]; uy = A[nch]; uz = A[2 * nch]; }
         A[0] = 0.0; A[nch] = 0.0; A[2 * nch] = 0.0;
         macc[lvl] = 0;
       }
@@ -197,7 +201,12 @@ __global__ void __launch_bounds__(256) corr_update_kernel(CorrArgs a) {
         const double dx = D[(int64_t)(idx * 3 + 0) * nch];
         const double dy = D[(int64_t)(idx * 3 + 1) * nch];
         const double dz = D[(int64_t)(idx * 3 + 2) * nch];
-        Cc[(int64_t)j * nch] += vx * dx + vy * dy + vz * dz;
+        if (a.type == 0) {
+          Cc[(int64_t)j * nch] += vx * dx + vy * dy + vz * dz;
+        } else {
+          const double ex = vx - dx, ey = vy - dy, ez = vz - dz;
+          Cc[(int64_t)j * nch] += ex * ex + ey * ey + ez * ez;
+        }
       }
       ins[lvl] = (in + 1 == p) ? 0 : in + 1;
       nins[lvl] = nins[lvl] < p ? nins[lvl] + 1 : p;
@@ -243,6 +252,34 @@ __global__ void copy_probe_kernel(const float4* __restrict__ in, float4* __restr
 // (wave form, four register groups): out[0..n) inclusive sums, out[n] total.
 // form 0: four chains of n <= 64 positions (row form): chain r reads in[r*n ..] and
 // writes out[r*(n+1) ..].
+// Centre-of-mass sample of one chain state in a chosen geometry (0 row form, 1/2/3 wave
+// form G = 1/2/4): in[4i..4i+3] = Qx Qy Qz N of strand i, in[4n] = Nk; out = 3 floats.
+template <int CPW, int G>
+__device__ void com_probe(const float* in, float* out, int n) {
+  const LaneC<CPW> L;
+  Strands<G> C;
+#pragma unroll
+  for (int j = 0; j < 4 * G; ++j) {
+    const int p = L.pos(j);
+    const bool on = p < n;
+    C.qx[j] = on ? in[4 * p] : 0.0f;
+    C.qy[j] = on ? in[4 * p + 1] : 0.0f;
+    C.qz[j] = on ? in[4 * p + 2] : 0.0f;
+    C.n[j] = on ? (int)in[4 * p + 3] : 0;
+    C.q2[j] = 0.0f;
+    C.w[j] = 0.0f;
+  }
+  float sv[3];
+  com_sample<CPW, G>(C, n, in[4 * n], L, sv);
+  if (L.lane == 0) { out[0] = sv[0]; out[1] = sv[1]; out[2] = sv[2]; }
+}
+__global__ void __launch_bounds__(64) com_probe_kernel(const float* in, float* out, int n, int geom) {
+  if (geom == 0) com_probe<4, 1>(in, out, n);
+  else if (geom == 1) com_probe<1, 1>(in, out, n);
+  else if (geom == 2) com_probe<1, 2>(in, out, n);
+  else com_probe<1, 4>(in, out, n);
+}
+
 __global__ void __launch_bounds__(64) scan_probe_kernel(const float* in, float* out, int n, int form) {
   if (form == 1) {
     const LaneC<1> L;
@@ -315,12 +352,12 @@ hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStrea
   }
 }
 
-int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats) {
+int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool msd) {
   switch (geom) {
-    case 0: return kmc_occupancy_geom_0(plain, lds_floats);
-    case 1: return kmc_occupancy_geom_1(plain, lds_floats);
-    case 2: return kmc_occupancy_geom_2(plain, lds_floats);
-    case 3: return kmc_occupancy_geom_3(plain, lds_floats);
+    case 0: return kmc_occupancy_geom_0(plain, lds_floats, msd);
+    case 1: return kmc_occupancy_geom_1(plain, lds_floats, msd);
+    case 2: return kmc_occupancy_geom_2(plain, lds_floats, msd);
+    case 3: return kmc_occupancy_geom_3(plain, lds_floats, msd);
     default: return 0;
   }
 }
@@ -367,9 +404,9 @@ hipError_t launch_flow_reduce(const float* stage, const int* stall, int chain0, 
 }
 
 hipError_t launch_corr_update(const float* stage, double* D, double* A, double* C, const int* stall, int chain0,
-                              int nch, int p, int m, int s, int nrows, const int* cnt, hipStream_t st) {
+                              int nch, int p, int m, int s, int nrows, const int* cnt, int type, hipStream_t st) {
   if (nrows <= 0 || nch <= 0) return hipSuccess;
-  CorrArgs a{stage, D, A, C, stall, chain0, nch, p, m, s, nrows, cnt};
+  CorrArgs a{stage, D, A, C, stall, chain0, nch, p, m, s, nrows, cnt, type};
   hipLaunchKernelGGL(corr_update_kernel, dim3((nch + 255) / 256), dim3(256), 0, st, a);
   return hipGetLastError();
 }
@@ -388,6 +425,10 @@ hipError_t launch_copy_probe(const void* in, void* out, size_t n4, int grid, hip
 }
 hipError_t launch_scan_probe(const float* in, float* out, int n, int form, hipStream_t st) {
   hipLaunchKernelGGL(scan_probe_kernel, dim3(1), dim3(64), 0, st, in, out, n, form);
+  return hipGetLastError();
+}
+hipError_t launch_com_probe(const float* in, float* out, int n, int geom, hipStream_t st) {
+  hipLaunchKernelGGL(com_probe_kernel, dim3(1), dim3(64), 0, st, in, out, n, geom);
   return hipGetLastError();
 }
 hipError_t launch_math_probe(const float* in, float* out, int n, int which, hipStream_t st) {

@@ -167,6 +167,27 @@ __device__ __forceinline__ float rdl(float v, int l) {
 }
 __device__ __forceinline__ int rdl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// Wave form, across a 256-position register-group boundary.  wave_prev_x: lane l > 0
+// gets cur from lane l-1, lane 0 gets prevg from lane 63; wave_next_x: lane l < 63 gets
+// cur from lane l+1, lane 63 gets nextg from lane 0.  The per-lane choice of the source
+// register is made before the permute, so no lane-dependent condition guards a
+// cross-lane read (a read of lane 63's value under "lane == 0" is not to be simplified
+// with facts about lane 0).
+__device__ __forceinline__ int wave_prev_x(int cur, int prevg, int lane) {
+  const int t = (lane == 63) ? prevg : cur;
+  return __builtin_amdgcn_ds_bpermute(((lane + 63) & 63) << 2, t);
+}
+__device__ __forceinline__ float wave_prev_x(float cur, float prevg, int lane) {
+  return __int_as_float(wave_prev_x(__float_as_int(cur), __float_as_int(prevg), lane));
+}
+__device__ __forceinline__ int wave_next_x(int cur, int nextg, int lane) {
+  const int t = (lane == 0) ? nextg : cur;
+  return __builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2, t);
+}
+__device__ __forceinline__ float wave_next_x(float cur, float nextg, int lane) {
+  return __int_as_float(wave_next_x(__float_as_int(cur), __float_as_int(nextg), lane));
+}
+
 // value of v on segment lane `src` (per lane; 0 <= src < SEG)
 template <int CPW>
 __device__ __forceinline__ float seg_get(float v, int src, const LaneC<CPW>& L) {
@@ -290,9 +311,8 @@ __device__ __forceinline__ void gap_open(T (&a)[4 * G], int at, int Z, const Lan
   T prev3[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    T pv = seg_prev<CPW, T>(a[4 * g + 3]);
-    if (CPW == 1 && g > 0) pv = (L.lane == 0) ? rdl(a[4 * g - 1], 63) : pv;
-    prev3[g] = pv;
+    prev3[g] = (CPW == 1 && g > 0) ? wave_prev_x(a[4 * g + 3], a[g > 0 ? 4 * g - 1 : 0], L.lane)
+                                   : seg_prev<CPW, T>(a[4 * g + 3]);
   }
 #pragma unroll
   for (int j = 4 * G - 1; j >= 0; --j) {
@@ -307,15 +327,26 @@ __device__ __forceinline__ void gap_close(T (&a)[4 * G], int at, int Z, const La
   T next0[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    T nv = seg_next<CPW, T>(a[4 * g]);
-    if (CPW == 1 && g + 1 < G) nv = (L.lane == 63) ? rdl(a[4 * g + 4 < 4 * G ? 4 * g + 4 : 0], 0) : nv;
-    next0[g] = nv;
+    next0[g] = (CPW == 1 && g + 1 < G) ? wave_next_x(a[4 * g], a[4 * g + 4 < 4 * G ? 4 * g + 4 : 0], L.lane)
+                                       : seg_next<CPW, T>(a[4 * g]);
   }
 #pragma unroll
   for (int j = 0; j < 4 * G; ++j) {
     const int p = L.pos(j);
     const T src = ((j & 3) != 3) ? a[j + 1 < 4 * G ? j + 1 : j] : next0[j >> 2];
     a[j] = (p >= at && p < Z - 1) ? src : a[j];
+  }
+}
+
+// out[p] = a[p-1], out[0] = 0 (whole wave)
+template <int CPW, int G>
+__device__ __forceinline__ void shift_prev(const float (&a)[4 * G], float (&out)[4 * G], const LaneC<CPW>& L) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    out[4 * g] = (CPW == 1 && g > 0) ? wave_prev_x(a[4 * g + 3], a[g > 0 ? 4 * g - 1 : 0], L.lane)
+                                     : seg_prev<CPW, float>(a[4 * g + 3]);
+#pragma unroll
+    for (int k = 1; k < 4; ++k) out[4 * g + k] = a[4 * g + k - 1];
   }
 }
 
@@ -434,15 +465,10 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
   float qn[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    int a = seg_next<CPW, int>(C.n[4 * g]);
-    float b = seg_next<CPW, float>(C.q2[4 * g]);
-    if (CPW == 1 && g + 1 < G) {
-      const int gn = g + 1 < G ? g + 1 : g;
-      a = (L.lane == 63) ? rdl(C.n[4 * gn], 0) : a;
-      b = (L.lane == 63) ? rdl(C.q2[4 * gn], 0) : b;
-    }
-    nn[g] = a;
-    qn[g] = b;
+    const int gn = g + 1 < G ? g + 1 : g;
+    const bool cross = CPW == 1 && g + 1 < G;
+    nn[g] = cross ? wave_next_x(C.n[4 * g], C.n[4 * gn], L.lane) : seg_next<CPW, int>(C.n[4 * g]);
+    qn[g] = cross ? wave_next_x(C.q2[4 * g], C.q2[4 * gn], L.lane) : seg_next<CPW, float>(C.q2[4 * g]);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -590,22 +616,57 @@ __device__ __forceinline__ void flow_sample(const Strands<G>& C, const Tabs<LT>&
   out[7] = (float)Z;
 }
 
+// MSD sample: centre of mass relative to R1 (common/eq_ensemble_kernel.cu:297-323),
+// sum_i (P_i + Q_i/2) N_i / Nk with P_i the canonical inclusive scan of Q at i-1;
+// the caller adds R1.
+template <int CPW, int G>
+__device__ __forceinline__ void com_sample(const Strands<G>& C, int Z, float nkf, const LaneC<CPW>& L,
+                                           float out[3]) {
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    float q[4 * G];
+#pragma unroll
+    for (int j = 0; j < 4 * G; ++j) {
+      const float x = comp == 0 ? C.qx[j] : comp == 1 ? C.qy[j] : C.qz[j];
+      q[j] = L.pos(j) < Z ? x : 0.0f;
+    }
+    float I[4 * G], P[4 * G], v[4 * G];
+    chain_scan<CPW, G>(q, I, Z, L);
+    shift_prev<CPW, G>(I, P, L);
+#pragma unroll
+    for (int j = 0; j < 4 * G; ++j)
+      v[j] = L.pos(j) < Z ? ((P[j] + q[j] * 0.5f) * (float)C.n[j]) / nkf : 0.0f;
+    out[comp] = chain_total<CPW, G>(v, Z, L);
+  }
+}
+
 // Record every pending sample of every chain of the wave: G(t) sample k while
 // t > k*delta (the state after the first jump past k*delta); flow output k while
 // t >= k*delta.
-template <int CPW, int G, bool LT>
+template <int CPW, int G, bool LT, bool MSD>
 __device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<LT>& T, const EnsLocal& E, Hdr& H,
-                                               float* stage, const LaneC<CPW>& L) {
-  if (E.sample_mode == kSampleEq) {
+                                               float* stage, const float* r1, const LaneC<CPW>& L) {
+  if (E.sample_mode == kSampleEq || (MSD && E.sample_mode == kSampleMsd)) {
+    const bool msd = MSD && E.sample_mode == kSampleMsd;
     for (;;) {
       const bool pend = !H.inert && H.t > H.t_next;
       if (!__ballot(pend)) break;
       float sv[3];
-      eq_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+      if constexpr (MSD) {
+        if (msd) com_sample<CPW, G>(C, H.Z, (float)H.nk, L, sv);
+        else eq_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+      } else {
+        eq_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+      }
       if (pend) {
         if (L.sl == 0 && H.next_sample >= E.k0) {
           float* o = stage + E.stage_off + ((H.next_sample - E.k0) * E.nchains + H.local) * 3;
-          o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+          if (MSD && msd) {
+            const float* r = r1 + 3 * (int64_t)H.c;
+            o[0] = sv[0] + r[0]; o[1] = sv[1] + r[1]; o[2] = sv[2] + r[2];
+          } else {
+            o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
+          }
         }
         ++H.next_sample;
         H.t_next = next_time(E, H.next_sample);
@@ -631,9 +692,9 @@ __device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<L
 }
 
 // ------------------------------------------------------------------ one jump of every active chain
-template <int CPW, int G, bool LT>
+template <int CPW, int G, bool LT, bool MSD>
 __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stage<CPW, G>& S, const Tabs<LT>& T,
-                                     const EnsLocal& E, RandCache& RC, int& k_rng, const LaneC<CPW>& L) {
+                                     const EnsLocal& E, RandCache& RC, int& k_rng, float* r1, const LaneC<CPW>& L) {
   constexpr int SEG = Geo<CPW, G>::SEG;
   constexpr int CAP = Geo<CPW, G>::CAP;
   const int Z = H.Z;
@@ -736,6 +797,19 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
     }
   }
   ++k_rng;
+  // MSD: destroying the first entanglement moves R1 forward by the old Q_1
+  // (common/eq_ensemble_kernel.cu:447-455,645-653)
+  if (MSD && E.sample_mode == kSampleMsd) {
+    const bool fwd = (kind == kEvCDd && p == 0) || kind == kEvSDdHead;
+    if (__ballot(fwd)) {
+      const float x = value_at<CPW, G>(C.qx, 1, L), y = value_at<CPW, G>(C.qy, 1, L);
+      const float z = value_at<CPW, G>(C.qz, 1, L);
+      if (fwd && L.sl == 0) {
+        float* r = r1 + 3 * (int64_t)H.c;
+        r[0] = r[0] + x; r[1] = r[1] + y; r[2] = r[2] + z;
+      }
+    }
+  }
   // 4. effect.  Common case: an N shift across pair p.
   int lo = 1, hi = 0, zero_at = -1;
   if (kind == kEvShiftP || kind == kEvShiftM) {
@@ -869,6 +943,12 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
       if (ps == w0i) C.w[j] = w0;
       if (ps == w1i) C.w[j] = w1;
     }
+    // MSD: a first entanglement created before the old one moves R1 back by the new Q_1
+    // (common/eq_ensemble_kernel.cu:531-539,606-614)
+    if (MSD && E.sample_mode == kSampleMsd && L.sl == 0 && ((kind == kEvCDc && p == 0) || kind == kEvSDcHead)) {
+      float* r = r1 + 3 * (int64_t)H.c;
+      r[0] = r[0] - p1x; r[1] = r[1] - p1y; r[2] = r[2] - p1z;
+    }
     H.Z = znew;
     stage_qw<CPW, G>(S, C, L);
   }
@@ -939,7 +1019,7 @@ __device__ __forceinline__ void store_strands(const Strands<G>& C, const ChainAr
 // Runs the chains of one work unit (four chains in row form, one in wave form) until
 // each reaches the ensemble's t_end (or its jump target ev_stop, or one jump in the
 // plain form), pushing chains that outgrow the geometry to the overflow list.
-template <int CPW, int G, bool LT, bool PLAIN>
+template <int CPW, int G, bool LT, bool PLAIN, bool MSD>
 __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg, lds_f* ltab, int& lds_ens) {
   constexpr int CAP = Geo<CPW, G>::CAP;
   constexpr int SEG = Geo<CPW, G>::SEG;
@@ -1004,12 +1084,12 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg,
       }
       H.dt_prev = 0.0f;
     }
-    record_samples<CPW, G, LT>(C, T, E, H, a.stage, L);
+    record_samples<CPW, G, LT, MSD>(C, T, E, H, a.stage, a.ch.R1, L);
     bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0 && !(PLAIN && H.done >= 1);
     if (act && H.Z >= CAP) { H.ovf = 1; act = false; }
     H.inert = H.inert || (H.ovf != 0);
     if (!__ballot(act)) break;
-    jump<CPW, G, LT>(C, H, act, S, T, E, RC, k_rng, L);
+    jump<CPW, G, LT, MSD>(C, H, act, S, T, E, RC, k_rng, a.ch.R1, L);
   }
   // write back every chain that was loaded live
   if (H.c >= 0 && !inert0) {
@@ -1031,7 +1111,9 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 
 // Persistent kernel: each wave takes work units from the queue until it is empty.
 // PLAIN: one jump per chain (the per-launch form); otherwise run to the targets.
-template <int CPW, int G, bool LT, bool PLAIN>
+// MSD: also tracks R1 and records centre-of-mass samples (the variant launched when an
+// ensemble of the pass samples MSD(t); it runs every other mode as well).
+template <int CPW, int G, bool LT, bool PLAIN, bool MSD>
 __global__ void __launch_bounds__(64) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
@@ -1043,7 +1125,7 @@ __global__ void __launch_bounds__(64) kmc_kernel(KmcArgs a) {
     if (lane_id() == 0) unit = atomicAdd(a.queue, 1);
     unit = __builtin_amdgcn_readfirstlane(unit);
     if (unit >= n_units) break;
-    run_unit<CPW, G, LT, PLAIN>(a, unit, stg, ltab, lds_ens);
+    run_unit<CPW, G, LT, PLAIN, MSD>(a, unit, stg, ltab, lds_ens);
   }
 }
 

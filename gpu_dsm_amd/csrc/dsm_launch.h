@@ -10,14 +10,14 @@ namespace dsm {
 // KMC kernels by chain geometry: 0 = row form (four chains per wave, Z <= 64),
 // 1/2/3 = wave form (one chain per wave, Z <= 256/512/1024).  Persistent: grid =
 // blocks of one wave; plain = one jump per chain.  lds_floats > 0 stages the rate
-// tables (that many floats) in LDS.
+// tables (that many floats) in LDS.  a.msd selects the variant with MSD(t) tracking.
 constexpr int kNumGeoms = 4;
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st);
-int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats);
+int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool msd);
 int kmc_geom_capacity(int geom);  // positions per chain
 #define DSM_DECLARE_GEOM(G)                                                                   \
   hipError_t launch_kmc_geom_##G(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
-  int kmc_occupancy_geom_##G(bool plain, int lds_floats);
+  int kmc_occupancy_geom_##G(bool plain, int lds_floats, bool msd);
 DSM_DECLARE_GEOM(0)
 DSM_DECLARE_GEOM(1)
 DSM_DECLARE_GEOM(2)
@@ -34,11 +34,12 @@ hipError_t launch_set_ev_stop(const uint64_t* ev, uint64_t* ev_stop, int chain0,
 hipError_t launch_flow_reduce(const float* stage, const int* stall, int chain0, int nch, int nrows, double* out,
                               hipStream_t st);
 hipError_t launch_corr_update(const float* stage, double* D, double* A, double* C, const int* stall, int chain0,
-                              int nch, int p, int m, int s, int nrows, const int* cnt, hipStream_t st);
+                              int nch, int p, int m, int s, int nrows, const int* cnt, int type, hipStream_t st);
 hipError_t launch_corr_reduce(const double* C, const int* stall, int chain0, int nch, int npts, double* out_sum,
                               int* out_healthy, hipStream_t st);
 hipError_t launch_copy_probe(const void* in, void* out, size_t n4, int grid, hipStream_t st);
 hipError_t launch_scan_probe(const float* in, float* out, int n, int form, hipStream_t st);
+hipError_t launch_com_probe(const float* in, float* out, int n, int geom, hipStream_t st);
 hipError_t launch_math_probe(const float* in, float* out, int n, int which, hipStream_t st);
 hipError_t launch_philox_probe(const uint32_t* ck, uint32_t* out, int n, hipStream_t st);
 

@@ -3,7 +3,8 @@
 // Chain state lives in HBM as structure-of-arrays.  Per chain (length n_chains):
 //   Z, nk, ens, cap, stall, cid (Philox chain id within its ensemble),
 //   local (index within its ensemble), base (offset of strand 0),
-//   t (FP64 clock), ev (event counter), dt_prev (FP32, flow), next_sample, ev_stop, zsum.
+//   t (FP64 clock), ev (event counter), dt_prev (FP32, flow), next_sample, ev_stop, zsum,
+//   R1 (3 floats: MSD fixed-frame position of the first entanglement).
 // Per strand slot (length total capacity, chain c owns [base[c], base[c]+cap[c])):
 //   Qx, Qy, Qz (FP32), N (int32), w (FP32 CD destruction rate of the link to
 //   the right of the strand).
@@ -15,7 +16,7 @@
 namespace dsm {
 
 constexpr int kNumTables = 9;  // SP SM PP PM EC ED ED2 T3 CC, each nmax+2 floats; then FR
-enum SampleMode : int { kSampleNone = 0, kSampleEq = 1, kSampleFlow = 2 };
+enum SampleMode : int { kSampleNone = 0, kSampleEq = 1, kSampleFlow = 2, kSampleMsd = 3 };
 
 struct EnsParams {
   float beta;
@@ -57,6 +58,7 @@ struct ChainArrays {
   int64_t* next_sample;
   uint64_t* ev_stop;  // jump-count target of the current fixed-count pass
   uint64_t* zsum;     // sum over the chain's jumps of Z at the jump (byte model)
+  float* R1;          // [3*chain] MSD: first entanglement in the fixed frame
   float* Qx;
   float* Qy;
   float* Qz;
@@ -76,6 +78,7 @@ struct KmcArgs {
   float* stage;         // sample staging arena
   int use_ev_stop;      // stop each chain when ev reaches ch.ev_stop[c]
   int lds_floats;       // per-wave LDS table copy size (0: tables read from global memory)
+  int msd;              // an ensemble of the pass samples MSD(t): the kernel variant with R1 tracking
 };
 
 }  // namespace dsm

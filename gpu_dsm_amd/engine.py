@@ -13,17 +13,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libgpu_dsm.so")
 EXE_PATH = os.path.join(HERE, "_build", "gpu_DSM")
 
-SAMPLE = {"none": 0, "gt": 1, "flow": 2}
+SAMPLE = {"none": 0, "gt": 1, "flow": 2, "msd": 3}
 MODE = {"batched": 0, "plain": 1}
 TABLES = ["SP", "SM", "PP", "PM", "EC", "ED", "ED2", "T3", "CC", "FR"]
 SCALARS = ["g", "alpha", "tau_0", "tau_max", "tau_d", "c1", "cm1", "rcm1c1", "At", "Adt", "Bdt",
            "normdt", "s_At", "s_Ct", "s_Dt", "s_Adt", "s_Bdt", "s_Cdt", "s_Ddt", "s_taud_inv",
            "s_one_minus_g", "c_create", "cut_threshold", "beta"]
 EXPORTS = ["dsm_create", "dsm_destroy", "dsm_last_error", "dsm_mem_info", "dsm_add_ensemble",
-           "dsm_remove_ensemble", "dsm_init_equilibrium", "dsm_state_cap", "dsm_get_state", "dsm_set_state",
+           "dsm_remove_ensemble", "dsm_init_equilibrium", "dsm_state_cap", "dsm_get_state", "dsm_set_state", "dsm_get_r1",
            "dsm_advance", "dsm_advance_many", "dsm_advance_events", "dsm_flow_stress", "dsm_correlator",
            "dsm_last_samples", "dsm_tables", "dsm_ensemble_scalars", "dsm_probe_scan", "dsm_probe_math",
-           "dsm_probe_philox", "dsm_probe_copy", "dsm_main", "dsm_main_ex", "dsm_serve", "dsm_serve_plan"]
+           "dsm_probe_philox", "dsm_probe_com", "dsm_probe_copy", "dsm_main", "dsm_main_ex", "dsm_serve", "dsm_serve_plan"]
 
 
 class DsmError(RuntimeError):
@@ -104,6 +104,7 @@ def load_library():
                                 P(C.c_float), P(C.c_double), P(C.c_uint64), P(C.c_int32), P(C.c_float),
                                 P(C.c_int64)]
     L.dsm_set_state.argtypes = L.dsm_get_state.argtypes
+    L.dsm_get_r1.argtypes = [C.c_void_p, C.c_int32, P(C.c_float)]
     L.dsm_advance.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, P(C.c_int32), P(C.c_int32),
                               P(RunStats)]
     L.dsm_advance_many.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_double), C.c_int32, C.c_int32, P(C.c_int32),
@@ -117,6 +118,7 @@ def load_library():
     L.dsm_probe_scan.argtypes = [C.c_int32, P(C.c_float), P(C.c_float), C.c_int32, C.c_int32]
     L.dsm_probe_math.argtypes = [C.c_int32, P(C.c_float), P(C.c_float), C.c_int32, C.c_int32]
     L.dsm_probe_philox.argtypes = [C.c_int32, P(C.c_uint32), P(C.c_uint32), C.c_int32]
+    L.dsm_probe_com.argtypes = [C.c_int32, P(C.c_float), P(C.c_float), C.c_int32, C.c_int32]
     L.dsm_probe_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, P(C.c_double)]
     L.dsm_serve.argtypes = [C.c_int32, P(ServeConfig), P(ServeReport), P(C.c_double)]
     L.dsm_serve_plan.argtypes = [P(ServeConfig), C.c_int32, P(C.c_double), P(C.c_int32), P(C.c_float),
@@ -212,7 +214,12 @@ class Engine:
         _check(self._L.dsm_get_state(self.h, int(ens), int(cap), _p(Z, C.c_int32), _p(N, C.c_int32),
                                      _p(Q, C.c_float), _p(w, C.c_float), _p(t, C.c_double), _p(ev, C.c_uint64),
                                      _p(st, C.c_int32), _p(dt, C.c_float), _p(ns, C.c_int64)))
-        return dict(Z=Z, N=N, Q=Q, w=w, t=t, ev=ev, stall=st, dt_prev=dt, next_sample=ns)
+        out = dict(Z=Z, N=N, Q=Q, w=w, t=t, ev=ev, stall=st, dt_prev=dt, next_sample=ns)
+        if self.ens[ens]["sample"] == "msd":
+            r1 = np.zeros((n, 3), np.float32)
+            _check(self._L.dsm_get_r1(self.h, int(ens), _p(r1, C.c_float)))
+            out["R1"] = r1
+        return out
 
     def set_state(self, ens, Z, N, Q, w, t=None, ev=None, stall=None, dt_prev=None, next_sample=None):
         n = self.ens[ens]["n"]
@@ -278,7 +285,7 @@ class Engine:
         k0, rows = C.c_int64(), C.c_int64()
         _check(self._L.dsm_last_samples(self.h, int(ens), None, 0, C.byref(k0), C.byref(rows)))
         n = self.ens[ens]["n"]
-        comp = 3 if self.ens[ens]["sample"] == "gt" else 8
+        comp = 8 if self.ens[ens]["sample"] == "flow" else 3
         out = np.zeros((rows.value, n, comp), np.float32)
         if rows.value:
             _check(self._L.dsm_last_samples(self.h, int(ens), _p(out, C.c_float), out.size, C.byref(k0),
@@ -300,6 +307,20 @@ class Engine:
 
 
 # ---------------------------------------------------------------- building-block probes
+def probe_com(Q, N, geom, device=0):
+    """MSD centre-of-mass sample (relative to R1) of one chain state computed by the
+    KMC kernels' code in geometry `geom` (0 row form, 1/2/3 wave form G = 1/2/4)."""
+    Q = np.asarray(Q, np.float32).reshape(-1, 3)
+    N = np.asarray(N)
+    n = Q.shape[0]
+    inp = np.zeros(4 * n + 1, np.float32)
+    inp[:4 * n] = np.concatenate([Q, N[:, None].astype(np.float32)], axis=1).ravel()
+    inp[4 * n] = float(N.sum())
+    out = np.zeros(3, np.float32)
+    _check(load_library().dsm_probe_com(device, _p(inp, C.c_float), _p(out, C.c_float), n, int(geom)))
+    return out
+
+
 def probe_scan(values, form=1, device=0):
     """Canonical scan tree on the device.  form 1: one chain (len <= 1024) -> (inclusive
     sums, total).  form 0: values shaped (4, n<=64), four row-form chains -> (sums (4, n),

@@ -34,6 +34,7 @@ extern "C" {
 #define DSM_SAMPLE_NONE 0
 #define DSM_SAMPLE_GT 1       /* off-diagonal stress samples for G(t) (common/eq_ensemble_kernel.cu:281-296) */
 #define DSM_SAMPLE_FLOW 2     /* full stress every delta (common/ensemble_kernel.cu:504-538) */
+#define DSM_SAMPLE_MSD 3      /* centre-of-mass samples for MSD(t) (common/eq_ensemble_kernel.cu:297-323) */
 
 #define DSM_MODE_BATCHED 0    /* persistent, register-resident, Z-binned work queue */
 #define DSM_MODE_PLAIN 1      /* one jump per chain per launch, state in HBM */
@@ -101,6 +102,10 @@ int dsm_get_state(dsm_engine* e, int32_t ens, int32_t cap, int32_t* Z, int32_t* 
 int dsm_set_state(dsm_engine* e, int32_t ens, int32_t cap, const int32_t* Z, const int32_t* N, const float* Q,
                   const float* w, const double* t, const uint64_t* ev, const int32_t* stall,
                   const float* dt_prev, const int64_t* next_sample);
+/* MSD mode: R1[n*3], each chain's first entanglement in the fixed frame, the chain_head
+   R1 field (common/chain.h:41) that the reference keeps on the device (d_R1,
+   common/ensemble_call_block.cu:75,406,422).  Zero after dsm_init_equilibrium. */
+int dsm_get_r1(dsm_engine* e, int32_t ens, float* r1);
 
 /* Advances every chain of the ensemble until its clock reaches t_end, recording
    samples and updating the correlator / flow stress on the way.  run_flag (may be
@@ -144,6 +149,10 @@ int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* out24);
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form);
 int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which);
 int dsm_probe_philox(int32_t device_id, const uint32_t* ctr_key, uint32_t* out, int32_t n);
+/* MSD centre-of-mass sample of one chain state (relative to R1) in KMC geometry `geom`
+   (0 row form, n <= 64; 1/2/3 wave form, n <= 256/512/1024): in[4i..4i+3] = Qx Qy Qz N
+   of strand i, in[4n] = Nk; out = 3 floats. */
+int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom);
 /* Plain device-to-device copy of n_bytes (multiple of 16), timed over `reps` launches
    after one warm-up: the achievable-bandwidth and byte-counter reference (no
    counterpart in the reference). */

@@ -349,6 +349,7 @@ void orc_chain_init_eq(orc_chain* c, const orc_params* p, const uint32_t key[2],
   c->nk = nk; c->chain_id = chain_id; c->t = 0.0; c->ev = 0; c->dt_prev = 0.0f; c->stall = 0;
   c->next_sample = p->flow ? 1 : 0;  /* flow output k is at time k*delta, k >= 1 */
   c->last_event = 0;
+  c->R1[0] = c->R1[1] = c->R1[2] = 0.0f;  /* common/chain.cu:207 */
   for (int i = 0; i < c->cap + 2; ++i) {
     c->N[i] = 0; c->Qx[i] = c->Qy[i] = c->Qz[i] = c->w[i] = 0.0f;
   }
@@ -512,6 +513,9 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
         refresh_range(c, p, e - 1, e + 1);
       } else if (m == 0) {   /* CD destruction of link e: merge strands e, e+1 */
         const int Nm = c->N[e] + c->N[e + 1];
+        if (p->msd && e == 0) {  /* the second entanglement becomes the first (eq_ensemble_kernel.cu:447-455) */
+          c->R1[0] = c->R1[0] + c->Qx[1]; c->R1[1] = c->R1[1] + c->Qy[1]; c->R1[2] = c->R1[2] + c->Qz[1];
+        }
         float mx = 0.0f, my = 0.0f, mz = 0.0f;
         if (!(e == 0 || e == Z - 2)) {
           mx = c->Qx[e] + c->Qx[e + 1]; my = c->Qy[e] + c->Qy[e + 1]; mz = c->Qz[e] + c->Qz[e + 1];
@@ -536,6 +540,9 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
           set_strand(c, 0, newn, 0.0f, 0.0f, 0.0f);
           c->w[0] = wnew;
           set_strand(c, 1, N - newn, g[0] * sig, g[1] * sig, g[2] * sig);
+          if (p->msd) {  /* new first entanglement, one strand before the old (eq_ensemble_kernel.cu:531-539) */
+            c->R1[0] = c->R1[0] - c->Qx[1]; c->R1[1] = c->R1[1] - c->Qy[1]; c->R1[2] = c->R1[2] - c->Qz[1];
+          }
           refresh_range(c, p, 0, 1);
         } else {
           const float sig = sqrtf(((float)newn * (float)(N - newn)) / (3.0f * (float)N));
@@ -580,6 +587,9 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
         c->w[0] = p->cd ? wnew : 0.0f;
         if (Z == 1) set_strand(c, 1, N0 - 1, 0.0f, 0.0f, 0.0f);
         else set_strand(c, 1, N0 - 1, g[0] * sig, g[1] * sig, g[2] * sig);
+        if (p->msd) {  /* eq_ensemble_kernel.cu:606-614 */
+          c->R1[0] = c->R1[0] - c->Qx[1]; c->R1[1] = c->R1[1] - c->Qy[1]; c->R1[2] = c->R1[2] - c->Qz[1];
+        }
         refresh_range(c, p, 0, 1);
         kind = ORC_EV_SDC_HEAD;
       } else if (m == 2) {   /* SD creation at the tail */
@@ -597,6 +607,9 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
         kind = ORC_EV_SDC_TAIL;
       } else if (m == 3) {   /* SD destruction at the head: strands 0,1 merge into an end */
         const int N1 = c->N[1];
+        if (p->msd) {  /* eq_ensemble_kernel.cu:645-653 */
+          c->R1[0] = c->R1[0] + c->Qx[1]; c->R1[1] = c->R1[1] + c->Qy[1]; c->R1[2] = c->R1[2] + c->Qz[1];
+        }
         close_gap(c, 0);
         c->Z = Z - 1;
         set_strand(c, 0, N1 + 1, 0.0f, 0.0f, 0.0f);
@@ -676,6 +689,30 @@ void orc_chain_stress_full(const orc_chain* c, const orc_params* p, float out[8]
   free(v);
 }
 
+/* Centre of mass (common/eq_ensemble_kernel.cu:297-323): strand i sits at its midpoint
+   P_i + Q_i/2 with P_i the position of its left end relative to R1, weighted N_i/Nk.
+   FP32 in the canonical order: P_i is the canonical inclusive scan of Q at i-1 (0 at
+   i = 0), the weighted terms (P_i + Q_i*0.5)*N_i/Nk are summed by the canonical tree,
+   then R1 is added. */
+void orc_chain_com_sample(const orc_chain* c, float out[3]) {
+  const int Z = c->Z;
+  const float nk = (float)c->nk;
+  float* q = malloc(sizeof(float) * 3 * (size_t)(Z + 64));
+  float* incl = q + Z + 64;
+  float* v = incl + Z + 64;
+  const float* src[3] = {c->Qx, c->Qy, c->Qz};
+  for (int comp = 0; comp < 3; ++comp) {
+    for (int i = 0; i < Z; ++i) q[i] = src[comp][i];
+    scan_tree_total(q, Z, incl);
+    for (int i = 0; i < Z; ++i) {
+      const float P = i > 0 ? incl[i - 1] : 0.0f;
+      v[i] = ((P + q[i] * 0.5f) * (float)c->N[i]) / nk;
+    }
+    out[comp] = orc_tree_total(v, Z) + c->R1[comp];
+  }
+  free(q);
+}
+
 int64_t orc_chain_advance(orc_chain* c, const orc_params* p, const uint32_t key[2], double t_end,
                           int64_t max_events, double delta, int64_t k0, int64_t k_end, float* samples,
                           int64_t stride) {
@@ -689,7 +726,8 @@ int64_t orc_chain_advance(orc_chain* c, const orc_params* p, const uint32_t key[
       /* G(t) sample k: state after the first jump past k*delta (common/eq_ensemble_kernel.cu:256-331) */
       while (c->next_sample < k_end && c->t > (double)c->next_sample * delta) {
         float sv[3];
-        orc_chain_stress_sample(c, p, sv);
+        if (p->msd) orc_chain_com_sample(c, sv);
+        else orc_chain_stress_sample(c, p, sv);
         if (c->next_sample >= k0) {  /* rows k0..k_end-1 only, as the kernels */
           float* o = samples + (c->next_sample - k0) * stride;
           o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
@@ -735,6 +773,7 @@ void orc_chain64_init_eq(orc_chain64* c, const orc_params* p, const uint32_t key
   c->nk = nk; c->chain_id = chain_id; c->t = 0.0; c->ev = 0; c->dt_prev = 0.0; c->stall = 0;
   c->next_sample = p->flow ? 1 : 0;
   c->last_event = 0;
+  c->R1[0] = c->R1[1] = c->R1[2] = 0.0;
   for (int i = 0; i < c->cap + 2; ++i) { c->N[i] = 0; c->Qx[i] = c->Qy[i] = c->Qz[i] = c->w[i] = 0.0; }
   c->Z = init_strands(c->N, p, key, chain_id, nk);
   for (int k = 1; k < c->Z - 1; ++k) {
@@ -847,6 +886,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
     if (pr >= tot) { pr -= tot; continue; }
     if (pr < wd) {
       const int Nm = c->N[e] + c->N[e + 1];
+      if (p->msd && e == 0) { c->R1[0] += c->Qx[1]; c->R1[1] += c->Qy[1]; c->R1[2] += c->Qz[1]; }
       double mx = 0, my = 0, mz = 0;
       if (!(e == 0 || e == Z - 2)) { mx = c->Qx[e] + c->Qx[e + 1]; my = c->Qy[e] + c->Qy[e + 1]; mz = c->Qz[e] + c->Qz[e + 1]; }
       const double w_next = c->w[e + 1];
@@ -874,6 +914,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
       set64(c, 0, newn, 0, 0, 0);
       c->w[0] = wnew;
       set64(c, 1, N - newn, g[0] * sig, g[1] * sig, g[2] * sig);
+      if (p->msd) { c->R1[0] -= c->Qx[1]; c->R1[1] -= c->Qy[1]; c->R1[2] -= c->Qz[1]; }
     } else {
       const double sig = sqrt((double)newn * (double)(N - newn) / (3.0 * N));
       const double ratio = (double)newn / N;
@@ -907,6 +948,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
       set64(c, 0, 1, 0, 0, 0);
       c->w[0] = p->cd ? orc_pcd_rate_ft_d(&p->pcd, u_tau) : 0.0;
       set64(c, 1, N0 - 1, g[0] * sig, g[1] * sig, g[2] * sig);
+      if (p->msd) { c->R1[0] -= c->Qx[1]; c->R1[1] -= c->Qy[1]; c->R1[2] -= c->Qz[1]; }
       kind = ORC_EV_SDC_HEAD;
     } else if ((pr -= endr[1]) < endr[2]) {
       NEED_G64();
@@ -920,6 +962,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
       kind = ORC_EV_SDC_TAIL;
     } else if ((pr -= endr[2]) < endr[3]) {
       const int N1 = c->N[1];
+      if (p->msd) { c->R1[0] += c->Qx[1]; c->R1[1] += c->Qy[1]; c->R1[2] += c->Qz[1]; }
       for (int i = 0; i < Z - 1; ++i) mv64(c, i, i + 1);
       c->Z = Z - 1;
       set64(c, 0, N1 + 1, 0, 0, 0);
@@ -947,6 +990,17 @@ void orc_chain64_stress_sample(const orc_chain64* c, double out[3]) {
     d -= 3.0 * c->Qx[i] * c->Qz[i] / N;
   }
   out[0] = a; out[1] = b; out[2] = d;
+}
+
+void orc_chain64_com_sample(const orc_chain64* c, double out[3]) {
+  double P[3] = {0, 0, 0}, s[3] = {0, 0, 0};
+  const double* q[3] = {c->Qx, c->Qy, c->Qz};
+  for (int i = 0; i < c->Z; ++i)
+    for (int k = 0; k < 3; ++k) {
+      s[k] += (P[k] + 0.5 * q[k][i]) * c->N[i] / c->nk;
+      P[k] += q[k][i];
+    }
+  for (int k = 0; k < 3; ++k) out[k] = s[k] + c->R1[k];
 }
 
 void orc_chain64_stress_full(const orc_chain64* c, double out[8]) {
@@ -986,7 +1040,8 @@ int64_t orc_chain64_advance(orc_chain64* c, const orc_params* p, const uint32_t 
     if (samples && !p->flow_samples) {
       while (c->next_sample < k_end && c->t > (double)c->next_sample * delta) {
         double sv[3];
-        orc_chain64_stress_sample(c, sv);
+        if (p->msd) orc_chain64_com_sample(c, sv);
+        else orc_chain64_stress_sample(c, sv);
         if (c->next_sample >= k0) {
           double* o = samples + (c->next_sample - k0) * stride;
           o[0] = sv[0]; o[1] = sv[1]; o[2] = sv[2];
@@ -1021,7 +1076,7 @@ int orc_corr_levels(double sim_time, double delta, int p, int m) {
 
 int orc_corr_init(orc_corr* cr, int p, int m, int s, int nchains) {
   memset(cr, 0, sizeof(*cr));
-  cr->p = p; cr->m = m; cr->s = s; cr->nchains = nchains;
+  cr->p = p; cr->m = m; cr->s = s; cr->nchains = nchains; cr->type = 0;
   cr->ins = calloc(s, sizeof(int)); cr->macc = calloc(s, sizeof(int));
   cr->nins = calloc(s, sizeof(int64_t));
   cr->D = calloc((size_t)nchains * s * p * 3, sizeof(double));
@@ -1030,6 +1085,8 @@ int orc_corr_init(orc_corr* cr, int p, int m, int s, int nchains) {
   cr->n = calloc((size_t)s * p, sizeof(double));
   return (cr->D && cr->A && cr->C) ? 0 : -1;
 }
+
+void orc_corr_set_type(orc_corr* cr, int type) { cr->type = type; }
 
 void orc_corr_free(orc_corr* cr) {
   free(cr->ins); free(cr->macc); free(cr->nins); free(cr->D); free(cr->A); free(cr->C); free(cr->n);
@@ -1044,14 +1101,19 @@ static void corr_add_level(orc_corr* cr, double* w, int k) {
   for (int c = 0; c < nc; ++c) {
     double* D = cr->D + (((size_t)c * s + k) * p) * 3;
     double* A = cr->A + ((size_t)c * s + k) * 3;
-    for (int q = 0; q < 3; ++q) { D[ins * 3 + q] = w[c * 3 + q]; A[q] += w[c * 3 + q]; }
+    /* type 1 (MSD) carries the first value of each block up, unaveraged
+       (common/eq_ensemble_kernel.cu:181-189) */
+    for (int q = 0; q < 3; ++q) {
+      D[ins * 3 + q] = w[c * 3 + q];
+      if (cr->type == 0 || cr->macc[k] == 0) A[q] += w[c * 3 + q];
+    }
   }
   cr->macc[k]++;
   if (cr->macc[k] == m) {
     double* up = malloc(sizeof(double) * 3 * (size_t)nc);
     for (int c = 0; c < nc; ++c) {
       double* A = cr->A + ((size_t)c * s + k) * 3;
-      for (int q = 0; q < 3; ++q) { up[c * 3 + q] = A[q] / m; A[q] = 0.0; }
+      for (int q = 0; q < 3; ++q) { up[c * 3 + q] = cr->type == 0 ? A[q] / m : A[q]; A[q] = 0.0; }
     }
     cr->macc[k] = 0;
     corr_add_level(cr, up, k + 1);
@@ -1066,7 +1128,13 @@ static void corr_add_level(orc_corr* cr, double* w, int k) {
     double sum = 0.0;
     for (int c = 0; c < nc; ++c) {
       const double* D = cr->D + (((size_t)c * s + k) * p) * 3;
-      sum += D[ins * 3 + 0] * D[idx * 3 + 0] + D[ins * 3 + 1] * D[idx * 3 + 1] + D[ins * 3 + 2] * D[idx * 3 + 2];
+      if (cr->type == 0) {
+        sum += D[ins * 3 + 0] * D[idx * 3 + 0] + D[ins * 3 + 1] * D[idx * 3 + 1] + D[ins * 3 + 2] * D[idx * 3 + 2];
+      } else {  /* squared displacement (common/eq_ensemble_kernel.cu:203,218) */
+        const double ex = D[ins * 3 + 0] - D[idx * 3 + 0], ey = D[ins * 3 + 1] - D[idx * 3 + 1];
+        const double ez = D[ins * 3 + 2] - D[idx * 3 + 2];
+        sum += ex * ex + ey * ey + ez * ez;
+      }
     }
     cr->C[k * p + j] += sum;
     cr->n[k * p + j] += 1.0;
@@ -1093,7 +1161,8 @@ int orc_corr_result(const orc_corr* cr, double* lag, double* g) {
     for (int j = jmin; j < cr->p; ++j) {
       const double n = cr->n[k * cr->p + j];
       lag[im] = (double)j * mk;
-      g[im] = n > 0 ? cr->C[k * cr->p + j] / n / 3.0 / cr->nchains : 0.0;
+      /* G: trace/3 (common/correlator.cu:176); MSD: the plain sum of the three components */
+      g[im] = n > 0 ? cr->C[k * cr->p + j] / n / (cr->type == 0 ? 3.0 : 1.0) / cr->nchains : 0.0;
       ++im;
     }
   }
@@ -1170,6 +1239,7 @@ orc_params* orc_params_new(float beta, int nk, int nmax, int cd, const float* ka
 void orc_params_delete(orc_params* p) { if (p) { orc_params_free(p); free(p); } }
 
 void orc_params_set_flow_samples(orc_params* p, int on) { p->flow_samples = on ? 1 : 0; }
+void orc_params_set_msd(orc_params* p, int on) { p->msd = on ? 1 : 0; }
 
 int orc_params_table(const orc_params* p, int which, float* out) {
   const float* t = NULL;
@@ -1264,6 +1334,11 @@ int64_t orc_ens_advance(orc_ens* e, double t_end, int64_t max_events, double del
 }
 
 int orc_ens_cap(const orc_ens* e) { return e->cap; }
+
+void orc_ens_get_r1(const orc_ens* e, float* r1) {
+  for (int i = 0; i < e->n; ++i)
+    for (int k = 0; k < 3; ++k) r1[i * 3 + k] = e->fp64 ? (float)e->c64[i].R1[k] : e->c32[i].R1[k];
+}
 
 void orc_ens_get(const orc_ens* e, int* Z, int* N, float* Q, float* w, double* t, uint64_t* ev, int* stall) {
   const int cap = e->cap;

@@ -19,7 +19,9 @@
  * error bounds against libm double, closed-form equilibrium distributions
  * (binomial Z, uniform compositions N|Z, Gaussian Q), detailed balance of
  * every sliding-dynamics move, p_CD closed forms (normalisation, <1/tau>),
- * correlator against brute-force autocorrelation, stationarity of long runs.
+ * correlator against brute-force autocorrelation (stress and MSD forms), conservation
+ * of the fixed-frame entanglement positions under every event (MSD R1 tracking),
+ * stationarity of long runs.
  */
 #ifndef DSM_ORACLE_H
 #define DSM_ORACLE_H
@@ -79,6 +81,7 @@ typedef struct {
   int flow;        /* any kappa != 0 */
   int exp_wait;    /* 0: dt = 1/W (reference); 1: dt = -ln(u)/W */
   int flow_samples;/* advance records 8-component flow stress (t >= k*delta) instead of G(t) samples */
+  int msd;         /* MSD(t): track R1 and record centre-of-mass samples instead of stress */
   float kappa[9];  /* xx xy xz yx yy yz zx zy zz */
   orc_pcd pcd;
   float c_create;  /* CD creation prefactor normdt/beta (common/ensemble.cu:136-150) */
@@ -102,6 +105,7 @@ typedef struct {
   int *N;
   float *Qx, *Qy, *Qz, *w;
   float *Q2, *Wp, *Wm, *R;   /* cached per-pair rates: pure functions of the state */
+  float R1[3];               /* MSD: first entanglement in the fixed frame (common/chain.h:41) */
   int last_event;
 } orc_chain;
 
@@ -120,6 +124,8 @@ float orc_tree_total(const float* v, int n);
 void orc_chain_stress_sample(const orc_chain* c, const orc_params* p, float out[3]);
 /* flow stress: xx yy zz xy yz zx Lpp Z */
 void orc_chain_stress_full(const orc_chain* c, const orc_params* p, float out[8]);
+/* MSD sample: centre of mass, sum_i (sum_{k<i} Q_k + Q_i/2) N_i / Nk + R1 */
+void orc_chain_com_sample(const orc_chain* c, float out[3]);
 void orc_chain_deform(orc_chain* c, const orc_params* p);
 
 /* advance until t >= t_end (or max_events jumps); records samples k
@@ -138,6 +144,7 @@ typedef struct {
   uint32_t chain_id;
   int *N;
   double *Qx, *Qy, *Qz, *w;
+  double R1[3];
   int last_event;
 } orc_chain64;
 int orc_chain64_alloc(orc_chain64* c, int cap);
@@ -145,6 +152,7 @@ void orc_chain64_free(orc_chain64* c);
 void orc_chain64_init_eq(orc_chain64* c, const orc_params* p, const uint32_t key[2], uint32_t chain_id, int nk);
 int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]);
 void orc_chain64_stress_sample(const orc_chain64* c, double out[3]);
+void orc_chain64_com_sample(const orc_chain64* c, double out[3]);
 void orc_chain64_stress_full(const orc_chain64* c, double out[8]);
 int64_t orc_chain64_advance(orc_chain64* c, const orc_params* p, const uint32_t key[2], double t_end,
                             int64_t max_events, double delta, int64_t k0, int64_t k_end, double* samples,
@@ -156,6 +164,7 @@ double orc_chain64_end_rates(const orc_chain64* c, const orc_params* p, double o
 /* ---------------- multi-tau correlator (common/eq_ensemble_kernel.cu:133-227) ---------------- */
 typedef struct {
   int p, m, s, nchains;
+  int type;      /* 0: stress autocorrelation, block averages; 1: MSD, (a-b)^2, first value of a block */
   int64_t nsamples;
   /* per level counters (identical for all chains: sampling is synchronous) */
   int *ins, *macc;
@@ -166,6 +175,7 @@ typedef struct {
   double *n;     /* [level][p] */
 } orc_corr;
 int orc_corr_init(orc_corr* cr, int p, int m, int s, int nchains);
+void orc_corr_set_type(orc_corr* cr, int type);
 void orc_corr_free(orc_corr* cr);
 /* add one synchronous sample row: w[chain*3 + comp] */
 void orc_corr_add_row(orc_corr* cr, const double* w);
@@ -184,6 +194,7 @@ typedef struct orc_ens orc_ens;
 orc_params* orc_params_new(float beta, int nk, int nmax, int cd, const float* kappa, int exp_wait);
 void orc_params_delete(orc_params* p);
 void orc_params_set_flow_samples(orc_params* p, int on);
+void orc_params_set_msd(orc_params* p, int on);
 float orc_params_rate_ft(const orc_params* p, float u);
 float orc_params_rate_fdt(const orc_params* p, float u);
 /* which: 0 SP 1 SM 2 PP 3 PM 4 EC 5 ED 6 ED2 7 T3 8 CC 9 FR; returns element count */
@@ -197,6 +208,7 @@ void orc_ens_delete(orc_ens* e);
 int64_t orc_ens_advance(orc_ens* e, double t_end, int64_t max_events, double delta, int64_t k0,
                         int64_t k_end, void* samples, int n_threads);
 /* state export: Z[n], N[n*cap], Q[n*cap*3], w[n*cap] (float; fp64 rounds), t[n], ev[n], stall[n] */
+void orc_ens_get_r1(const orc_ens* e, float* r1);  /* [n*3]; fp64 rounds */
 int orc_ens_cap(const orc_ens* e);
 void orc_ens_get(const orc_ens* e, int* Z, int* N, float* Q, float* w, double* t, uint64_t* ev, int* stall);
 void orc_ens_get64(const orc_ens* e, double* Q, double* w);

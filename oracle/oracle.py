@@ -53,6 +53,8 @@ def lib():
         L.orc_params_new.restype = C.c_void_p
         L.orc_params_delete.argtypes = [C.c_void_p]
         L.orc_params_set_flow_samples.argtypes = [C.c_void_p, C.c_int]
+        L.orc_params_set_msd.argtypes = [C.c_void_p, C.c_int]
+        L.orc_ens_get_r1.argtypes = [C.c_void_p, f32p]
         L.orc_params_table.argtypes = [C.c_void_p, C.c_int, f32p]; L.orc_params_table.restype = C.c_int
         L.orc_params_scalars.argtypes = [C.c_void_p, f32p]
         L.orc_params_rate_ft.argtypes = [C.c_void_p, C.c_float]; L.orc_params_rate_ft.restype = C.c_float
@@ -211,12 +213,15 @@ class Ensemble:
             self.h = None
 
     def advance(self, t_end=float("inf"), max_events=2**62, delta=1.0, k0=0, k_end=0, samples=False,
-                threads=None, flow_samples=False):
+                threads=None, flow_samples=False, msd=False):
         """Advance every chain; returns (events, samples[k][chain][ncomp] or None).
 
-        ncomp is 3 (G(t) samples) or 8 (flow_samples: xx yy zz xy yz zx Lpp Z)."""
+        ncomp is 3 (G(t) stress samples, or with msd=True the centre of mass) or 8
+        (flow_samples: xx yy zz xy yz zx Lpp Z).  msd=True also tracks R1, the first
+        entanglement's fixed-frame position, through the jumps."""
         threads = threads or min(os.cpu_count() or 1, 16)
         lib().orc_params_set_flow_samples(self.params.h, int(bool(flow_samples)))
+        lib().orc_params_set_msd(self.params.h, int(bool(msd)))
         buf = None
         if samples and k_end > k0:
             buf = np.zeros((k_end - k0, self.n, 8 if flow_samples else 3), np.float64 if self.fp64 else np.float32)
@@ -231,7 +236,9 @@ class Ensemble:
         t = np.zeros(n, np.float64); ev = np.zeros(n, np.uint64); st = np.zeros(n, np.int32)
         lib().orc_ens_get(self.h, _p(Z, C.c_int), _p(N, C.c_int), _p(Q, C.c_float), _p(w, C.c_float),
                           _p(t, C.c_double), _p(ev, C.c_uint64), _p(st, C.c_int))
-        out = dict(Z=Z, N=N, Q=Q, w=w, t=t, ev=ev, stall=st)
+        r1 = np.zeros((n, 3), np.float32)
+        lib().orc_ens_get_r1(self.h, _p(r1, C.c_float))
+        out = dict(Z=Z, N=N, Q=Q, w=w, t=t, ev=ev, stall=st, R1=r1)
         if self.fp64:
             Q64 = np.zeros((n, cap, 3)); w64 = np.zeros((n, cap))
             lib().orc_ens_get64(self.h, _p(Q64, C.c_double), _p(w64, C.c_double))
@@ -285,10 +292,13 @@ class Correlator:
 
     Follows common/eq_ensemble_kernel.cu:133-227 (insert, block average,
     lag products) and common/correlator.cu:104-187 (result extraction).
+    type 0: stress autocorrelation (block averages go up a level, products, trace/3);
+    type 1: MSD (the first value of each block goes up unaveraged, squared
+    differences, sum of the three components).
     """
 
-    def __init__(self, p, m, s, nchains):
-        self.p, self.m, self.s, self.nc = p, m, s, nchains
+    def __init__(self, p, m, s, nchains, type=0):
+        self.p, self.m, self.s, self.nc, self.type = p, m, s, nchains, int(type)
         self.D = np.zeros((s, p, nchains, 3))
         self.A = np.zeros((s, nchains, 3))
         self.ins = np.zeros(s, np.int64)
@@ -303,10 +313,11 @@ class Correlator:
         p, m = self.p, self.m
         ins = self.ins[k]
         self.D[k, ins] = w
-        self.A[k] += w
+        if self.type == 0 or self.macc[k] == 0:
+            self.A[k] += w
         self.macc[k] += 1
         if self.macc[k] == m:
-            up = self.A[k] / m
+            up = self.A[k] / m if self.type == 0 else self.A[k].copy()
             self.A[k] = 0.0
             self.macc[k] = 0
             self._add(up, k + 1)
@@ -316,7 +327,11 @@ class Correlator:
             if j > have:
                 continue
             idx = (ins - j) % p
-            self.C[k, j] += float(np.sum(self.D[k, ins] * self.D[k, idx]))
+            if self.type == 0:
+                self.C[k, j] += float(np.sum(self.D[k, ins] * self.D[k, idx]))
+            else:
+                d = self.D[k, ins] - self.D[k, idx]
+                self.C[k, j] += float(np.sum(d * d))
             self.n[k, j] += 1.0
         self.ins[k] = (ins + 1) % p
         self.nins[k] += 1
@@ -331,5 +346,5 @@ class Correlator:
             for j in range(jmin, self.p):
                 n = self.n[k, j]
                 lags.append(j * self.m ** k)
-                g.append(self.C[k, j] / n / 3.0 / self.nc if n > 0 else 0.0)
+                g.append(self.C[k, j] / n / (3.0 if self.type == 0 else 1.0) / self.nc if n > 0 else 0.0)
         return np.array(lags, np.float64), np.array(g)

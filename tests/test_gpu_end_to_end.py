@@ -77,24 +77,28 @@ def test_cli_flow_run_matches_oracle(tmp_path, oracle):
         assert d.ndim == 2 and d.shape[0] > 1 and np.all(np.isfinite(d))
 
 
-def test_cli_gt_run_matches_oracle(tmp_path, oracle):
+@pytest.mark.parametrize("eq_type,name", [(1, "G"), (2, "MSD")])
+def test_cli_gt_run_matches_oracle(tmp_path, oracle, eq_type, name):
+    """equilibrium_type 1 writes G_<job>.dat, 2 writes MSD_<job>.dat (common/main_cuda.cu:142-155,
+    common/ensemble.cu:324-335)."""
     beta, nk, n, T, dt, job = 1.0, 20, 128, 100.0, 1.0, 7
     p, m = 16, 2
-    _write_input(tmp_path / "input.dat", beta, nk, n, [0] * 9, 1, 1, dt, T)
-    _run(tmp_path, str(job), "--corr-p", str(p), "--corr-m", str(m), "--quiet")
-    g = np.loadtxt(tmp_path / ("G_%d.dat" % job))
+    _write_input(tmp_path / "input.dat", beta, nk, n, [0] * 9, 1, eq_type, dt, T)
+    out = _run(tmp_path, str(job), "--corr-p", str(p), "--corr-m", str(m), "--quiet")
+    assert ("%s(t) calc..." % name) in out
+    g = np.loadtxt(tmp_path / ("%s_%d.dat" % (name, job)))
     s = int(np.ceil(np.log(T / dt / p) / np.log(m))) + 1
     P = oracle.Params(beta, nk, 1)
     O = oracle.Ensemble(P, seed=job, n=n, salt=0)
-    _, samples = O.advance(t_end=T, delta=dt, k0=0, k_end=int(T), samples=True)
-    corr = oracle.Correlator(p, m, s, n)
+    _, samples = O.advance(t_end=T, delta=dt, k0=0, k_end=int(T), samples=True, msd=eq_type == 2)
+    corr = oracle.Correlator(p, m, s, n, type=eq_type - 1)
     for row in samples:
         corr.add_row(row)
     lag_o, g_o = corr.result()
     k = g.shape[0]
     assert 0 < k <= len(lag_o)
     np.testing.assert_allclose(g[:, 0], lag_o[:k])
-    np.testing.assert_allclose(g[:, 1], g_o[:k], rtol=2e-5, atol=1e-6 * abs(g_o[0]))
+    np.testing.assert_allclose(g[:, 1], g_o[:k], rtol=2e-5, atol=1e-6 * abs(g_o[0 if eq_type == 1 else 1]))
 
 
 def test_cli_load_rejects_size_mismatch(tmp_path):

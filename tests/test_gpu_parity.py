@@ -187,6 +187,72 @@ def test_gt_samples_and_correlator(engine, oracle):
     engine.remove_ensemble(ens)
 
 
+@pytest.mark.parametrize("Z,geom", [(1, 0), (2, 0), (60, 0), (64, 0), (65, 1), (250, 1), (256, 1), (257, 2),
+                                     (268, 2), (500, 2), (512, 2), (513, 3), (900, 3), (1024, 3)])
+def test_com_sample_matches_oracle(oracle, Z, geom):
+    """The centre-of-mass sample of the kernels, in every geometry, against the oracle."""
+    import gpu_dsm_amd as g
+    rng = np.random.default_rng(Z)
+    N = rng.integers(1, 4, Z)
+    Q = rng.normal(size=(Z, 3)).astype(np.float32)
+    if Z > 1:
+        Q[0] = 0.0
+        Q[Z - 1] = 0.0
+    nk = int(N.sum())
+    E = oracle.Ensemble(oracle.Params(1.0, nk, 0), seed=1, n=1)
+    cap = E.cap
+    Nf = np.zeros((1, cap), np.int32); Nf[0, :Z] = N
+    Qf = np.zeros((1, cap, 3), np.float32); Qf[0, :Z] = Q
+    E.set_state(np.array([Z]), Nf, Qf, np.zeros((1, cap), np.float32), t=np.array([0.5]),
+                next_sample=np.zeros(1, np.int64))
+    _, smp = E.advance(max_events=0, delta=1.0, k0=0, k_end=1, samples=True, msd=True)
+    got = g.probe_com(Q, N, geom)
+    assert np.array_equal(_bits(got), _bits(smp[0, 0])), (got, smp[0, 0])
+
+
+MSD_CASES = [
+    dict(n=200, nk=12, beta=1.0, cd=1, T=300.0, mode="batched"),   # short chains: head events every few jumps
+    dict(n=127, nk=40, beta=0.5, cd=1, T=150.0, mode="batched"),
+    dict(n=64, nk=12, beta=1.0, cd=0, T=40.0, mode="plain"),
+    dict(n=9, nk=400, beta=0.5, cd=1, T=15.0, mode="batched"),     # wave form (Z up to ~290)
+]
+
+
+@pytest.mark.parametrize("case", MSD_CASES, ids=lambda c: "nk%d_b%g_cd%d_%s" % (c["nk"], c["beta"], c["cd"], c["mode"]))
+def test_msd_samples_and_correlator(engine, oracle, case):
+    """MSD(t): R1 bookkeeping, centre-of-mass samples (bit-exact) and the type-1 multi-tau
+    correlator against the oracle."""
+    n, nk, beta, cd, T = case["n"], case["nk"], case["beta"], case["cd"], case["T"]
+    p, m = 16, 2
+    s = oracle.corr_levels(T, 1.0, p, m)
+    P = oracle.Params(beta, nk, cd)
+    ens = engine.add_ensemble(n, nk, beta, cd, seed=8, sample="msd", delta=1.0, corr_p=p, corr_m=m,
+                              corr_levels=s, window=64)
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=8, n=n)
+    _, samples = O.advance(t_end=T, delta=1.0, k0=0, k_end=int(T), samples=True, msd=True)
+    engine.advance(ens, T, mode=case["mode"])
+    g_st, o_st = engine.state(ens), O.state()
+    assert_same_state(g_st, o_st, "MSD run")
+    assert np.array_equal(_bits(g_st["R1"]), _bits(o_st["R1"])), "R1 differs"
+    assert np.abs(o_st["R1"]).max() > 0
+    k0, last = engine.last_samples(ens)
+    ref = samples[k0:k0 + last.shape[0]]
+    bad = np.argwhere(_bits(last) != _bits(ref))
+    assert bad.size == 0, "%d sample values differ; first (row, chain, comp) %s, gpu %s oracle %s, Z %s" % (
+        bad.shape[0], bad[:4].tolist(), [float(last[tuple(b)]) for b in bad[:4]],
+        [float(ref[tuple(b)]) for b in bad[:4]], g_st["Z"][np.unique(bad[:, 1])][:6].tolist())
+    corr = oracle.Correlator(p, m, s, n, type=1)
+    for row in samples:
+        corr.add_row(row)
+    lag_o, g_o = corr.result()
+    lag, g = engine.correlator(ens)
+    assert np.array_equal(lag, lag_o)
+    np.testing.assert_allclose(g, g_o, rtol=1e-10, atol=1e-12)
+    assert g[0] == 0.0 and g[1] > 0
+    engine.remove_ensemble(ens)
+
+
 def test_flow_stress_rows(engine, oracle):
     n, nk, T, dlt = 128, 30, 20.0, 2.0
     kappa = [0, 0.02, 0, 0, 0, 0, 0, 0, 0]
