@@ -22,6 +22,7 @@ constexpr uint32_t kPurposeInitCut = 16u;
 constexpr uint32_t kPurposeInitQ = 17u;
 constexpr uint32_t kPurposeInitTau = 18u;
 constexpr uint32_t kPurposeInitNk = 19u;
+constexpr uint32_t kPurposeInitPartner = 20u;  // partner quantile of each initial link (top 16 bits)
 
 enum EventKind : int {
   kEvNone = 0, kEvShiftP = 1, kEvShiftM = 2, kEvCDd = 3, kEvCDc = 4, kEvCDcTail = 5,

@@ -229,9 +229,14 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
 
   bool flow = false;
   for (float k : in.kappa) flow = flow || (k != 0.0f);
+  // common/main_cuda.cu:95-103: PD_flag reads the GEX parameters a, b, mp, Mk
+  float gex[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   if (in.PD_flag) {
-    std::cout << "PD_flag=1 (GEX polydisperse CD background from polydisp.dat) is not supported by this build\n";
-    return DSM_ERR_ARG;
+    std::ifstream pf(dir.empty() ? "polydisp.dat" : dir + "/polydisp.dat");
+    if (!(pf >> gex[0] >> gex[1] >> gex[2] >> gex[3])) {
+      std::cout << "cannot read polydisp.dat\n";
+      return DSM_ERR_IO;
+    }
   }
   dsm_engine* e = nullptr;
   int rc = dsm_create(device_ID, &e);
@@ -256,6 +261,10 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
   d.corr_m = o.corr_m;
   d.corr_levels = s;
   d.window = o.window;
+  if (in.PD_flag) {
+    d.partner_mode = DSM_PARTNER_GEX;
+    for (int i = 0; i < 4; ++i) d.gex[i] = gex[i];
+  }
   int ens = 0;
   if ((rc = dsm_add_ensemble(e, &d, &ens))) {
     std::cout << "error: " << dsm_last_error() << "\n";

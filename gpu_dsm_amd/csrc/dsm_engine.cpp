@@ -84,6 +84,115 @@ void pcd_init(Pcd& p, float Be, float Nk) {
 
 double pow075(double x) { return std::sqrt(x) * std::sqrt(std::sqrt(x)); }
 
+// ------------------------------------------------------------------ CD partner lengths
+// Polydisperse CD background: the chain on the other side of an entanglement has a
+// length drawn from the weight distribution of the melt (common/gpu_random.cu:62-76,
+// 212-228, common/chain.cu:141-149).  The distribution is held as kPartnerK equiprobable
+// quantiles (the Nk at probability (k+1/2)/K), each with its p_CD sampler constants;
+// a draw uses the top kPartnerBits bits of a Philox word as the quantile index.
+
+// Regularised lower incomplete gamma P(s, x): the power series of gamma(s, x) for
+// x < s + 1, else 1 - Q from Legendre's continued fraction (modified Lentz).
+double gamma_p(double s, double x) {
+  if (x <= 0.0) return 0.0;
+  const double lead = -x + s * std::log(x) - std::lgamma(s);
+  if (x < s + 1.0) {
+    double term = 1.0 / s, sum = term;
+    for (int n = 1; n < 100000; ++n) {
+      term *= x / (s + n);
+      sum += term;
+      if (std::fabs(term) < std::fabs(sum) * 1e-17) break;
+    }
+    return sum * std::exp(lead);
+  }
+  const double tiny = 1e-300;
+  double b = x + 1.0 - s, c = 1.0 / tiny, d = 1.0 / b, h = d;
+  for (int i = 1; i < 100000; ++i) {
+    const double an = -i * (i - s);
+    b += 2.0;
+    d = an * d + b;
+    if (std::fabs(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (std::fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (std::fabs(del - 1.0) < 1e-16) break;
+  }
+  return 1.0 - std::exp(lead) * h;
+}
+
+// Nk of the K quantiles of the partner weight distribution.  mode 1: GEX, weight
+// fraction W(m) = P((a+1)/b, m^b) of the reduced mass m, Nk = m*mp/Mk
+// (common/gamma.cpp:174-263, polydisp.dat); mode 2: the ensemble's own lengths, each
+// distinct Nk weighted by Nk times its chain count.
+bool partner_quantiles(int mode, const float gex[4], const std::vector<int>& nk_chain, std::vector<float>& out) {
+  out.assign(kPartnerK, 0.0f);
+  if (mode == 1) {
+    const double a = gex[0], b = gex[1], mp = gex[2], Mk = gex[3];
+    if (!(a > -1.0) || !(b > 0.0) || !(mp > 0.0) || !(Mk > 0.0)) return false;
+    const double s = (a + 1.0) / b;
+    for (int k = 0; k < kPartnerK; ++k) {
+      const double y = (k + 0.5) / kPartnerK;
+      double lo = 0.0, hi = 1.0;
+      while (gamma_p(s, hi) < y) hi *= 2.0;
+      for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (gamma_p(s, mid) < y) lo = mid; else hi = mid;
+      }
+      const double m = std::pow(0.5 * (lo + hi), 1.0 / b);
+      out[k] = (float)(m * mp / Mk);
+    }
+    return true;
+  }
+  if (mode == 2) {
+    std::map<int, double> w;
+    for (int n : nk_chain) w[n] += (double)n;
+    double tot = 0.0;
+    for (auto& kv : w) tot += kv.second;
+    auto it = w.begin();
+    double cum = it->second;
+    for (int k = 0; k < kPartnerK; ++k) {
+      const double y = (k + 0.5) / kPartnerK * tot;
+      while (cum < y && std::next(it) != w.end()) { ++it; cum += it->second; }
+      out[k] = (float)it->first;
+    }
+    return true;
+  }
+  return false;
+}
+
+// Partner tables of one ensemble: for SD-created and initial links (f_t) the K
+// quantiles above; for CD-created links (f_d_t) the same entries re-weighted by their
+// mean destruction rate normdt (common/gamma.cpp:236-262).  Layout: [F consts 5K]
+// [D consts 5K][F Nk K][D Nk K]; F consts s_At s_Ct s_Dt s_taud_inv s_one_minus_g,
+// D consts s_Adt s_Bdt s_Cdt s_Ddt s_taud_inv.  Returns the CD creation prefactor
+// <normdt>_F / beta (common/ensemble.cu:137-146).
+float partner_tables(const std::vector<float>& nkF, float beta, std::vector<float>& t) {
+  const int K = kPartnerK;
+  t.assign((size_t)12 * K, 0.0f);
+  std::vector<double> cum(K);
+  std::vector<Pcd> pc(K);
+  double tot = 0.0;
+  for (int k = 0; k < K; ++k) {
+    pcd_init(pc[k], beta, nkF[k]);
+    tot += (double)pc[k].normdt;
+    cum[k] = tot;
+    float* f = &t[(size_t)5 * k];
+    f[0] = pc[k].s_At; f[1] = pc[k].s_Ct; f[2] = pc[k].s_Dt; f[3] = pc[k].s_taud_inv; f[4] = pc[k].s_one_minus_g;
+    t[(size_t)10 * K + k] = nkF[k];
+  }
+  int i = 0;
+  for (int j = 0; j < K; ++j) {
+    const double y = (j + 0.5) / K * tot;
+    while (cum[i] < y && i + 1 < K) ++i;
+    float* d = &t[(size_t)5 * K + (size_t)5 * j];
+    d[0] = pc[i].s_Adt; d[1] = pc[i].s_Bdt; d[2] = pc[i].s_Cdt; d[3] = pc[i].s_Ddt; d[4] = pc[i].s_taud_inv;
+    t[(size_t)11 * K + j] = nkF[i];
+  }
+  return (float)(tot / K) / beta;
+}
+
 // FP32 rate tables, indexed by the integer strand length N (see dsm_kernels.hip pair_rate).
 void build_tables(std::vector<float>& t, int nmax, float beta, int cd, float c_create) {
   const int n = nmax + 2;
@@ -415,8 +524,8 @@ struct Timer {
   }
 };
 
-int device_blocks(dsm_engine* e, int geom, bool plain, int lds_floats, bool msd) {
-  int nb = kmc_max_blocks_per_cu(geom, plain, lds_floats, msd);
+int device_blocks(dsm_engine* e, int geom, bool plain, int lds_floats, bool ext) {
+  int nb = kmc_max_blocks_per_cu(geom, plain, lds_floats, ext);
   if (nb <= 0) nb = 1;
   return nb * e->n_cu;
 }
@@ -452,9 +561,9 @@ struct Pass {
 };
 
 int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pass& ps) {
-  int n0 = 0, bin_max = 0, tab_max = 0, msd = 0;
+  int n0 = 0, bin_max = 0, tab_max = 0, ext = 0;
   for (int id : ids) {
-    msd |= e->ens[id].P.sample_mode == kSampleMsd ? 1 : 0;
+    ext |= (e->ens[id].P.sample_mode == kSampleMsd || e->ens[id].P.partner) ? 1 : 0;
     n0 += (e->ens[id].n + 3) & ~3;
     bin_max = std::max(bin_max, e->ens[id].bin_max);
     tab_max = std::max(tab_max, 9 * (e->ens[id].nmax + 2) + 4 * e->ens[id].nmax + 4);
@@ -472,7 +581,7 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   ps.a.use_ev_stop = use_ev_stop ? 1 : 0;
   // rate tables go to LDS (one copy per wave) when they fit the per-wave budget
   ps.a.lds_floats = tab_max <= kLdsTableFloats ? tab_max : 0;
-  ps.a.msd = msd;
+  ps.a.ext = ext;
   return DSM_OK;
 }
 
@@ -489,7 +598,7 @@ int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs
     a.queue = cnt + (kNumBins + 1) + b;
     a.overflow = e->d_work + (size_t)(b + 1) * e->d_work_cap;
     a.n_overflow = cnt + b + 1;
-    const int grid = device_blocks(e, b, plain, a.lds_floats, a.msd != 0);
+    const int grid = device_blocks(e, b, plain, a.lds_floats, a.ext != 0);
     if (timed) HIPCHK(hipEventRecord(e->evs[0], e->st));
     HIPCHK(launch_kmc(a, b, plain, grid, e->st));
     rs->kmc_launches++;
@@ -781,6 +890,16 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   const int nk_ref = d->nk_per_chain ? nmax : d->nk;
   Pcd pc;
   pcd_init(pc, d->beta, (float)nk_ref);
+  // polydisperse CD background: partner quantiles and their sampler constants
+  std::vector<float> pdt;
+  float c_partner = 0.0f;
+  if (d->partner_mode != DSM_PARTNER_OWN) {
+    std::vector<float> nkF;
+    if ((d->partner_mode != DSM_PARTNER_GEX && d->partner_mode != DSM_PARTNER_MELT) ||
+        !partner_quantiles(d->partner_mode, d->gex, E.nk, nkF))
+      return fail(DSM_ERR_ARG, "bad partner_mode or GEX parameters");
+    c_partner = partner_tables(nkF, d->beta, pdt);
+  }
   EnsParams& P = E.P;
   P.beta = d->beta;
   P.nk = nk_ref;
@@ -794,7 +913,9 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   P.exp_wait = d->exp_wait ? 1 : 0;
   P.sample_mode = d->sample_mode;
   P.nchains = E.n;
-  P.c_create = pc.normdt / d->beta;
+  P.c_create = d->partner_mode != DSM_PARTNER_OWN ? c_partner : pc.normdt / d->beta;
+  P.partner = d->partner_mode != DSM_PARTNER_OWN ? 1 : 0;
+  P.pd_off = 0;
   P.s_At = pc.s_At; P.s_Ct = pc.s_Ct; P.s_Dt = pc.s_Dt; P.s_Adt = pc.s_Adt; P.s_Bdt = pc.s_Bdt;
   P.s_Cdt = pc.s_Cdt; P.s_Ddt = pc.s_Ddt; P.s_taud_inv = pc.s_taud_inv; P.s_one_minus_g = pc.s_one_minus_g;
   {
@@ -822,6 +943,11 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
     if ((rc = upload_tabs(e))) return rc;
   }
   P.fr_off = P.tab_off + kNumTables * (nmax + 2);
+  if (P.partner) {
+    P.pd_off = (int)e->tabs.size();
+    e->tabs.insert(e->tabs.end(), pdt.begin(), pdt.end());
+    if ((rc = upload_tabs(e))) return rc;
+  }
   // sampling
   E.ncomp = (d->sample_mode == DSM_SAMPLE_GT || d->sample_mode == DSM_SAMPLE_MSD) ? 3
             : d->sample_mode == DSM_SAMPLE_FLOW ? 8 : 0;
@@ -896,7 +1022,7 @@ int dsm_init_equilibrium(dsm_engine* e, int32_t ens) {
   int rc = set_device(e);
   if (rc) return rc;
   HIPCHK(launch_iota(e->d_work, E->chain0, E->n, e->st));
-  HIPCHK(launch_init_eq(e->ch, e->d_ens, e->d_work, E->n, e->st));
+  HIPCHK(launch_init_eq(e->ch, e->d_ens, e->d_tabs, e->d_work, E->n, e->st));
   HIPCHK(hipMemsetAsync(e->ch.R1 + 3 * (int64_t)E->chain0, 0, 12 * (size_t)E->n, e->st));  // common/chain.cu:207
   HIPCHK(hipStreamSynchronize(e->st));
   E->k_cur = E->P.sample_mode == kSampleFlow ? 1 : 0;
@@ -1133,10 +1259,15 @@ int dsm_last_samples(dsm_engine* e, int32_t ens, float* out, int64_t max_floats,
 
 int dsm_tables(dsm_engine* e, int32_t ens, int32_t which, float* out, int32_t* n) {
   Ens* E = get_ens(e, ens);
-  if (!E || which < 0 || which > 9) return fail(DSM_ERR_ARG, "bad ensemble or table");
+  if (!E || which < 0 || which > 13) return fail(DSM_ERR_ARG, "bad ensemble or table");
+  if (which >= 10 && !E->P.partner) return fail(DSM_ERR_STATE, "ensemble %d has no partner tables", ens);
   const int nn = E->nmax + 2;
-  const int cnt = which == 9 ? 4 * E->nmax + 4 : nn;
-  const int off = which == 9 ? E->P.fr_off : E->P.tab_off + which * nn;
+  // 10/11: f_t / f_d_t partner sampler constants (5 per quantile); 12/13: their Nk
+  const int cnt = which == 9 ? 4 * E->nmax + 4 : which >= 12 ? kPartnerK : which >= 10 ? 5 * kPartnerK : nn;
+  const int off = which == 9    ? E->P.fr_off
+                  : which >= 12 ? E->P.pd_off + 10 * kPartnerK + (which - 12) * kPartnerK
+                  : which >= 10 ? E->P.pd_off + (which - 10) * 5 * kPartnerK
+                                : E->P.tab_off + which * nn;
   if (n) *n = cnt;
   if (out) std::memcpy(out, e->tabs.data() + off, sizeof(float) * cnt);
   return DSM_OK;

@@ -28,9 +28,10 @@ namespace dsm {
 // entanglement iff Philox word (i-1) < 2^32/(1+beta): Z-1 ~ Binomial(nk-1, 1/(1+beta))
 // and N|Z uniform over compositions (the laws of common/chain.cu:38-88).
 // Internal strands get Q ~ N(0, N/3) per component (common/chain.cu:90-102),
-// links get CD rates from f_t (common/chain.cu:155-158).
-__global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsParams* ens, const int* chains,
-                                                        int n) {
+// links get CD rates from f_t (common/chain.cu:155-158), with the partner length of
+// each link drawn from the background distribution when there is one (:141-149).
+__global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsParams* ens, const float* tabs,
+                                                        const int* chains, int n) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   const int c = chains[idx];
@@ -63,10 +64,18 @@ __global__ void __launch_bounds__(256) init_eq_kernel(ChainArrays ch, const EnsP
     ch.w[b + k] = 0.0f;
   }
   if (P.cd) {
+    const float* pd = tabs + P.pd_off;
+    uint4 y = make_uint4(0, 0, 0, 0);
     for (int k = 0; k < Z - 1; ++k) {
       if ((k & 3) == 0) x = philox4x32_10(make_uint4((uint32_t)(k >> 2), 0u, cid, kPurposeInitTau), k0, k1);
       const uint32_t word = (k & 3) == 0 ? x.x : (k & 3) == 1 ? x.y : (k & 3) == 2 ? x.z : x.w;
-      ch.w[b + k] = rate_ft(&P, u01(word));
+      if (P.partner) {
+        if ((k & 3) == 0) y = philox4x32_10(make_uint4((uint32_t)(k >> 2), 0u, cid, kPurposeInitPartner), k0, k1);
+        const uint32_t pw = (k & 3) == 0 ? y.x : (k & 3) == 1 ? y.y : (k & 3) == 2 ? y.z : y.w;
+        ch.w[b + k] = rate_ft_partner(pd, (int)(pw >> (32 - kPartnerBits)), u01(word));
+      } else {
+        ch.w[b + k] = rate_ft(&P, u01(word));
+      }
     }
   }
   ch.Z[c] = Z; ch.stall[c] = 0; ch.t[c] = 0.0; ch.ev[c] = 0; ch.dt_prev[c] = 0.0f; ch.zsum[c] = 0;
@@ -352,21 +361,22 @@ hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStrea
   }
 }
 
-int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool msd) {
+int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext) {
   switch (geom) {
-    case 0: return kmc_occupancy_geom_0(plain, lds_floats, msd);
-    case 1: return kmc_occupancy_geom_1(plain, lds_floats, msd);
-    case 2: return kmc_occupancy_geom_2(plain, lds_floats, msd);
-    case 3: return kmc_occupancy_geom_3(plain, lds_floats, msd);
+    case 0: return kmc_occupancy_geom_0(plain, lds_floats, ext);
+    case 1: return kmc_occupancy_geom_1(plain, lds_floats, ext);
+    case 2: return kmc_occupancy_geom_2(plain, lds_floats, ext);
+    case 3: return kmc_occupancy_geom_3(plain, lds_floats, ext);
     default: return 0;
   }
 }
 
 int kmc_geom_capacity(int geom) { return geom == 0 ? 64 : geom == 1 ? 256 : geom == 2 ? 512 : 1024; }
 
-hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int* chains, int n, hipStream_t st) {
+hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const float* tabs, const int* chains, int n,
+                          hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(init_eq_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ch, ens, chains, n);
+  hipLaunchKernelGGL(init_eq_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ch, ens, tabs, chains, n);
   return hipGetLastError();
 }
 

@@ -70,6 +70,17 @@ __device__ __forceinline__ float rate_ft(const EnsParams* P, float u) {
 __device__ __forceinline__ float rate_fdt(const EnsParams* P, float u) {
   return u < P->s_Bdt ? dpowf(__builtin_fmaf(u, P->s_Adt, P->s_Ddt), P->s_Cdt) : P->s_taud_inv;
 }
+// the same with the constants of partner quantile `idx` (polydisperse background,
+// common/gpu_random.cu:212-228): c = s_At s_Ct s_Dt s_taud_inv s_one_minus_g (f_t) or
+// s_Adt s_Bdt s_Cdt s_Ddt s_taud_inv (f_d_t)
+__device__ __forceinline__ float rate_ft_partner(const float* pd, int idx, float u) {
+  const float* c = pd + 5 * idx;
+  return u < c[4] ? dpowf(__builtin_fmaf(u, c[0], c[2]), c[1]) : c[3];
+}
+__device__ __forceinline__ float rate_fdt_partner(const float* pd, int idx, float u) {
+  const float* c = pd + 5 * (kPartnerK + idx);
+  return u < c[1] ? dpowf(__builtin_fmaf(u, c[0], c[3]), c[2]) : c[4];
+}
 
 // Sliding-dynamics shift rates of the pair (i, j=i+1) and the pair's total with its
 // CD terms (common/eq_ensemble_kernel.cu:97-126).
@@ -404,7 +415,7 @@ struct Hdr {
 // Per-wave ensemble values (every chain of a wave belongs to one ensemble).
 struct EnsLocal {
   const EnsParams* P;
-  int cd, flow, exp_wait, sample_mode, nchains, nmax;
+  int cd, flow, exp_wait, sample_mode, nchains, nmax, partner;
   uint32_t key0, key1;
   double delta, t_end;
   int64_t k0, k_end, stage_off;
@@ -416,6 +427,7 @@ __device__ __forceinline__ void load_ens(EnsLocal& E, const EnsParams* __restric
   E.P = &P;
   E.cd = P.cd; E.flow = P.flow; E.exp_wait = P.exp_wait; E.sample_mode = P.sample_mode;
   E.nchains = P.nchains; E.nmax = P.nmax; E.key0 = P.key0; E.key1 = P.key1; E.delta = P.delta;
+  E.partner = P.partner;
   E.t_end = P.t_end; E.k0 = P.k0; E.k_end = P.k_end; E.stage_off = P.stage_off; E.beta = P.beta;
 }
 
@@ -643,16 +655,16 @@ __device__ __forceinline__ void com_sample(const Strands<G>& C, int Z, float nkf
 // Record every pending sample of every chain of the wave: G(t) sample k while
 // t > k*delta (the state after the first jump past k*delta); flow output k while
 // t >= k*delta.
-template <int CPW, int G, bool LT, bool MSD>
+template <int CPW, int G, bool LT, bool EXT>
 __device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<LT>& T, const EnsLocal& E, Hdr& H,
                                                float* stage, const float* r1, const LaneC<CPW>& L) {
-  if (E.sample_mode == kSampleEq || (MSD && E.sample_mode == kSampleMsd)) {
-    const bool msd = MSD && E.sample_mode == kSampleMsd;
+  if (E.sample_mode == kSampleEq || (EXT && E.sample_mode == kSampleMsd)) {
+    const bool msd = EXT && E.sample_mode == kSampleMsd;
     for (;;) {
       const bool pend = !H.inert && H.t > H.t_next;
       if (!__ballot(pend)) break;
       float sv[3];
-      if constexpr (MSD) {
+      if constexpr (EXT) {
         if (msd) com_sample<CPW, G>(C, H.Z, (float)H.nk, L, sv);
         else eq_sample<CPW, G, LT>(C, T, H.Z, L, sv);
       } else {
@@ -661,7 +673,7 @@ __device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<L
       if (pend) {
         if (L.sl == 0 && H.next_sample >= E.k0) {
           float* o = stage + E.stage_off + ((H.next_sample - E.k0) * E.nchains + H.local) * 3;
-          if (MSD && msd) {
+          if (EXT && msd) {
             const float* r = r1 + 3 * (int64_t)H.c;
             o[0] = sv[0] + r[0]; o[1] = sv[1] + r[1]; o[2] = sv[2] + r[2];
           } else {
@@ -692,9 +704,10 @@ __device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<L
 }
 
 // ------------------------------------------------------------------ one jump of every active chain
-template <int CPW, int G, bool LT, bool MSD>
+template <int CPW, int G, bool LT, bool EXT>
 __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stage<CPW, G>& S, const Tabs<LT>& T,
-                                     const EnsLocal& E, RandCache& RC, int& k_rng, float* r1, const LaneC<CPW>& L) {
+                                     const EnsLocal& E, RandCache& RC, int& k_rng, float* r1, const float* gtabs,
+                                     const LaneC<CPW>& L) {
   constexpr int SEG = Geo<CPW, G>::SEG;
   constexpr int CAP = Geo<CPW, G>::CAP;
   const int Z = H.Z;
@@ -799,7 +812,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   ++k_rng;
   // MSD: destroying the first entanglement moves R1 forward by the old Q_1
   // (common/eq_ensemble_kernel.cu:447-455,645-653)
-  if (MSD && E.sample_mode == kSampleMsd) {
+  if (EXT && E.sample_mode == kSampleMsd) {
     const bool fwd = (kind == kEvCDd && p == 0) || kind == kEvSDdHead;
     if (__ballot(fwd)) {
       const float x = value_at<CPW, G>(C.qx, 1, L), y = value_at<CPW, G>(C.qy, 1, L);
@@ -834,6 +847,17 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
     const float g0 = seg_get<CPW>(RC.g0, k_rng - 1, L), g1 = seg_get<CPW>(RC.g1, k_rng - 1, L);
     const float g2 = seg_get<CPW>(RC.g2, k_rng - 1, L);
     const int N0n = S.geti(kStN, 0), N1n = S.geti(kStN, 1);
+    // polydisperse background: the new link's partner quantile is the top bits of word 3
+    // of this jump's Philox draw (H.ev already counts the jump)
+    int pidx = 0;
+    const float* pd = gtabs + E.P->pd_off;
+    const bool partner = EXT && E.partner;
+    if (partner) {
+      const uint64_t evj = H.ev - 1;
+      const uint4 x = philox4x32_10(make_uint4((uint32_t)evj, (uint32_t)(evj >> 32), H.cid, kPurposeStep), E.key0,
+                                    E.key1);
+      pidx = (int)(x.w >> (32 - kPartnerBits));
+    }
     int gap = 0, at = CAP, znew = Z;
     int p0 = -1, p0N = 0, p1 = -1, p1N = 0, w0i = -1, w1i = -1;
     float p0x = 0.0f, p0y = 0.0f, p0z = 0.0f, p1x = 0.0f, p1y = 0.0f, p1z = 0.0f, w0 = 0.0f, w1 = 0.0f;
@@ -853,7 +877,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
         const int Ne = Np;
         const float wc = T.CC(Ne);
         const int newn = split_newn(r, Ne, wc);
-        const float wnew = rate_fdt(E.P, u_tau);
+        const float wnew = partner ? rate_fdt_partner(pd, pidx, u_tau) : rate_fdt(E.P, u_tau);
         gap = 1; at = e; znew = Z + 1;
         w0i = e; w0 = wnew;
         if (e == 0) {
@@ -879,13 +903,13 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
         znew = Z + 1;
         p0 = Z - 1; p0N = newn;
         if (Z != 1) { p0x = g0 * sig; p0y = g1 * sig; p0z = g2 * sig; }
-        w0i = Z - 1; w0 = rate_fdt(E.P, u_tau);
+        w0i = Z - 1; w0 = partner ? rate_fdt_partner(pd, pidx, u_tau) : rate_fdt(E.P, u_tau);
         p1 = Z; p1N = N - newn;
         w1i = Z; w1 = 0.0f;
         lo = Z - 2; hi = Z - 1;
         H.dlo = min(H.dlo, Z - 1); H.dhi = max(H.dhi, Z);
       } else if (kind == kEvSDcHead) { // SD creation at the head
-        const float wnew = rate_ft(E.P, u_tau);
+        const float wnew = partner ? rate_ft_partner(pd, pidx, u_tau) : rate_ft(E.P, u_tau);
         const float sig = (Z == 1) ? 0.0f : sqrtf((float)(N0n - 1) / 3.0f);
         gap = 1; at = 0; znew = Z + 1;
         p0 = 0; p0N = 1;
@@ -895,7 +919,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
         lo = 0; hi = 1;
         H.dlo = min(H.dlo, 0); H.dhi = max(H.dhi, Z);
       } else if (kind == kEvSDcTail) { // SD creation at the tail
-        const float wnew = rate_ft(E.P, u_tau);
+        const float wnew = partner ? rate_ft_partner(pd, pidx, u_tau) : rate_ft(E.P, u_tau);
         const float sig = (Z == 1) ? 0.0f : sqrtf((float)(Nz - 1) / 3.0f);
         znew = Z + 1;
         p0 = Z - 1; p0N = Nz - 1;
@@ -945,7 +969,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
     }
     // MSD: a first entanglement created before the old one moves R1 back by the new Q_1
     // (common/eq_ensemble_kernel.cu:531-539,606-614)
-    if (MSD && E.sample_mode == kSampleMsd && L.sl == 0 && ((kind == kEvCDc && p == 0) || kind == kEvSDcHead)) {
+    if (EXT && E.sample_mode == kSampleMsd && L.sl == 0 && ((kind == kEvCDc && p == 0) || kind == kEvSDcHead)) {
       float* r = r1 + 3 * (int64_t)H.c;
       r[0] = r[0] - p1x; r[1] = r[1] - p1y; r[2] = r[2] - p1z;
     }
@@ -1019,7 +1043,7 @@ __device__ __forceinline__ void store_strands(const Strands<G>& C, const ChainAr
 // Runs the chains of one work unit (four chains in row form, one in wave form) until
 // each reaches the ensemble's t_end (or its jump target ev_stop, or one jump in the
 // plain form), pushing chains that outgrow the geometry to the overflow list.
-template <int CPW, int G, bool LT, bool PLAIN, bool MSD>
+template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
 __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg, lds_f* ltab, int& lds_ens) {
   constexpr int CAP = Geo<CPW, G>::CAP;
   constexpr int SEG = Geo<CPW, G>::SEG;
@@ -1084,12 +1108,12 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg,
       }
       H.dt_prev = 0.0f;
     }
-    record_samples<CPW, G, LT, MSD>(C, T, E, H, a.stage, a.ch.R1, L);
+    record_samples<CPW, G, LT, EXT>(C, T, E, H, a.stage, a.ch.R1, L);
     bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0 && !(PLAIN && H.done >= 1);
     if (act && H.Z >= CAP) { H.ovf = 1; act = false; }
     H.inert = H.inert || (H.ovf != 0);
     if (!__ballot(act)) break;
-    jump<CPW, G, LT, MSD>(C, H, act, S, T, E, RC, k_rng, a.ch.R1, L);
+    jump<CPW, G, LT, EXT>(C, H, act, S, T, E, RC, k_rng, a.ch.R1, a.tabs, L);
   }
   // write back every chain that was loaded live
   if (H.c >= 0 && !inert0) {
@@ -1111,9 +1135,10 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 
 // Persistent kernel: each wave takes work units from the queue until it is empty.
 // PLAIN: one jump per chain (the per-launch form); otherwise run to the targets.
-// MSD: also tracks R1 and records centre-of-mass samples (the variant launched when an
-// ensemble of the pass samples MSD(t); it runs every other mode as well).
-template <int CPW, int G, bool LT, bool PLAIN, bool MSD>
+// EXT: the variant with the less common features, launched when an ensemble of the pass
+// needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
+// centre-of-mass samples) and the polydisperse CD background (partner tables).
+template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
 __global__ void __launch_bounds__(64) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
@@ -1125,7 +1150,7 @@ __global__ void __launch_bounds__(64) kmc_kernel(KmcArgs a) {
     if (lane_id() == 0) unit = atomicAdd(a.queue, 1);
     unit = __builtin_amdgcn_readfirstlane(unit);
     if (unit >= n_units) break;
-    run_unit<CPW, G, LT, PLAIN, MSD>(a, unit, stg, ltab, lds_ens);
+    run_unit<CPW, G, LT, PLAIN, EXT>(a, unit, stg, ltab, lds_ens);
   }
 }
 

@@ -17,43 +17,43 @@ namespace {
 constexpr int kCPW = DSM_GEOM == 0 ? 4 : 1;
 constexpr int kG = DSM_GEOM == 0 ? 1 : (DSM_GEOM == 1 ? 1 : (DSM_GEOM == 2 ? 2 : 4));
 
-template <bool LT, bool PLAIN, bool MSD>
+template <bool LT, bool PLAIN, bool EXT>
 hipError_t launch_one(const KmcArgs& a, int grid, hipStream_t st) {
   const size_t lds = sizeof(float) * ((size_t)stage_floats<kCPW, kG>() + (LT ? (size_t)a.lds_floats : 0));
-  hipLaunchKernelGGL((kmc_kernel<kCPW, kG, LT, PLAIN, MSD>), dim3(grid), dim3(64), lds, st, a);
+  hipLaunchKernelGGL((kmc_kernel<kCPW, kG, LT, PLAIN, EXT>), dim3(grid), dim3(64), lds, st, a);
   return hipGetLastError();
 }
 
-template <bool LT, bool PLAIN, bool MSD>
+template <bool LT, bool PLAIN, bool EXT>
 int occupancy_one(int lds_floats) {
   int nb = 0;
   const size_t lds = sizeof(float) * ((size_t)stage_floats<kCPW, kG>() + (LT ? (size_t)lds_floats : 0));
   const hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_kernel<kCPW, kG, LT, PLAIN, MSD>, 64, lds);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_kernel<kCPW, kG, LT, PLAIN, EXT>, 64, lds);
   return e == hipSuccess ? nb : 0;
 }
 
-template <bool MSD>
+template <bool EXT>
 hipError_t launch_variant(const KmcArgs& a, bool plain, int grid, hipStream_t st) {
   const bool lt = a.lds_floats > 0;
-  if (plain) return lt ? launch_one<true, true, MSD>(a, grid, st) : launch_one<false, true, MSD>(a, grid, st);
-  return lt ? launch_one<true, false, MSD>(a, grid, st) : launch_one<false, false, MSD>(a, grid, st);
+  if (plain) return lt ? launch_one<true, true, EXT>(a, grid, st) : launch_one<false, true, EXT>(a, grid, st);
+  return lt ? launch_one<true, false, EXT>(a, grid, st) : launch_one<false, false, EXT>(a, grid, st);
 }
 
-template <bool MSD>
+template <bool EXT>
 int occupancy_variant(bool plain, int lds_floats) {
-  if (plain) return lds_floats > 0 ? occupancy_one<true, true, MSD>(lds_floats) : occupancy_one<false, true, MSD>(0);
-  return lds_floats > 0 ? occupancy_one<true, false, MSD>(lds_floats) : occupancy_one<false, false, MSD>(0);
+  if (plain) return lds_floats > 0 ? occupancy_one<true, true, EXT>(lds_floats) : occupancy_one<false, true, EXT>(0);
+  return lds_floats > 0 ? occupancy_one<true, false, EXT>(lds_floats) : occupancy_one<false, false, EXT>(0);
 }
 }  // namespace
 
 hipError_t DSM_CAT(launch_kmc_geom_, DSM_GEOM)(const KmcArgs& a, bool plain, int grid, hipStream_t st) {
   if (grid <= 0) return hipSuccess;
-  return a.msd ? launch_variant<true>(a, plain, grid, st) : launch_variant<false>(a, plain, grid, st);
+  return a.ext ? launch_variant<true>(a, plain, grid, st) : launch_variant<false>(a, plain, grid, st);
 }
 
-int DSM_CAT(kmc_occupancy_geom_, DSM_GEOM)(bool plain, int lds_floats, bool msd) {
-  return msd ? occupancy_variant<true>(plain, lds_floats) : occupancy_variant<false>(plain, lds_floats);
+int DSM_CAT(kmc_occupancy_geom_, DSM_GEOM)(bool plain, int lds_floats, bool ext) {
+  return ext ? occupancy_variant<true>(plain, lds_floats) : occupancy_variant<false>(plain, lds_floats);
 }
 
 }  // namespace dsm

@@ -10,20 +10,22 @@ namespace dsm {
 // KMC kernels by chain geometry: 0 = row form (four chains per wave, Z <= 64),
 // 1/2/3 = wave form (one chain per wave, Z <= 256/512/1024).  Persistent: grid =
 // blocks of one wave; plain = one jump per chain.  lds_floats > 0 stages the rate
-// tables (that many floats) in LDS.  a.msd selects the variant with MSD(t) tracking.
+// tables (that many floats) in LDS.  a.ext selects the variant with MSD(t) tracking and
+// polydisperse CD partners.
 constexpr int kNumGeoms = 4;
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st);
-int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool msd);
+int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext);
 int kmc_geom_capacity(int geom);  // positions per chain
 #define DSM_DECLARE_GEOM(G)                                                                   \
   hipError_t launch_kmc_geom_##G(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
-  int kmc_occupancy_geom_##G(bool plain, int lds_floats, bool msd);
+  int kmc_occupancy_geom_##G(bool plain, int lds_floats, bool ext);
 DSM_DECLARE_GEOM(0)
 DSM_DECLARE_GEOM(1)
 DSM_DECLARE_GEOM(2)
 DSM_DECLARE_GEOM(3)
 #undef DSM_DECLARE_GEOM
-hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const int* chains, int n, hipStream_t st);
+hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const float* tabs, const int* chains, int n,
+                          hipStream_t st);
 hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_iota(int* p, int start, int n, hipStream_t st);
 // out[0] += sum ev[chain0..chain0+n), out[1] += sum zsum[...]

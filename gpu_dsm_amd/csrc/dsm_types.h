@@ -16,6 +16,10 @@
 namespace dsm {
 
 constexpr int kNumTables = 9;  // SP SM PP PM EC ED ED2 T3 CC, each nmax+2 floats; then FR
+// CD partner lengths of a polydisperse background: equiprobable quantiles, indexed by
+// the top kPartnerBits bits of a Philox word
+constexpr int kPartnerBits = 16;
+constexpr int kPartnerK = 1 << kPartnerBits;
 enum SampleMode : int { kSampleNone = 0, kSampleEq = 1, kSampleFlow = 2, kSampleMsd = 3 };
 
 struct EnsParams {
@@ -35,6 +39,8 @@ struct EnsParams {
   int tab_off;       // float offset of SP in the table arena
   int fr_off;        // float offset of FR
   int chain0;        // first global chain index
+  int partner;       // 0: CD partners have this ensemble's Nk; 1: partner tables at pd_off
+  int pd_off;        // float offset of the partner tables (5K F consts, 5K D consts, K, K Nk)
   int pad0;
   double delta;      // sample / output interval
   double t_end;      // target time of the current run
@@ -78,7 +84,7 @@ struct KmcArgs {
   float* stage;         // sample staging arena
   int use_ev_stop;      // stop each chain when ev reaches ch.ev_stop[c]
   int lds_floats;       // per-wave LDS table copy size (0: tables read from global memory)
-  int msd;              // an ensemble of the pass samples MSD(t): the kernel variant with R1 tracking
+  int ext;              // an ensemble of the pass samples MSD(t) or has a polydisperse CD background
 };
 
 }  // namespace dsm

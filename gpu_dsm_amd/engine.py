@@ -15,7 +15,7 @@ EXE_PATH = os.path.join(HERE, "_build", "gpu_DSM")
 
 SAMPLE = {"none": 0, "gt": 1, "flow": 2, "msd": 3}
 MODE = {"batched": 0, "plain": 1}
-TABLES = ["SP", "SM", "PP", "PM", "EC", "ED", "ED2", "T3", "CC", "FR"]
+TABLES = ["SP", "SM", "PP", "PM", "EC", "ED", "ED2", "T3", "CC", "FR", "PF", "PD", "PF_NK", "PD_NK"]
 SCALARS = ["g", "alpha", "tau_0", "tau_max", "tau_d", "c1", "cm1", "rcm1c1", "At", "Adt", "Bdt",
            "normdt", "s_At", "s_Ct", "s_Dt", "s_Adt", "s_Bdt", "s_Cdt", "s_Ddt", "s_taud_inv",
            "s_one_minus_g", "c_create", "cut_threshold", "beta"]
@@ -34,7 +34,10 @@ class EnsembleDesc(C.Structure):
     _fields_ = [("n_chains", C.c_int32), ("nk", C.c_int32), ("nk_per_chain", C.POINTER(C.c_int32)),
                 ("beta", C.c_float), ("cd", C.c_int32), ("kappa", C.c_float * 9), ("exp_wait", C.c_int32),
                 ("seed", C.c_uint32), ("salt", C.c_uint32), ("sample_mode", C.c_int32), ("delta", C.c_double),
-                ("corr_p", C.c_int32), ("corr_m", C.c_int32), ("corr_levels", C.c_int32), ("window", C.c_int32)]
+                ("corr_p", C.c_int32), ("corr_m", C.c_int32), ("corr_levels", C.c_int32), ("window", C.c_int32),
+                ("partner_mode", C.c_int32), ("gex", C.c_float * 4)]
+
+PARTNER = {None: 0, "own": 0, "gex": 1, "melt": 2}
 
 
 class RunStats(C.Structure):
@@ -175,7 +178,11 @@ class Engine:
 
     # ------------------------------------------------------------ ensembles
     def add_ensemble(self, n_chains, nk, beta=1.0, cd=0, kappa=None, seed=1, salt=0, sample="none", delta=1.0,
-                     corr_p=64, corr_m=8, corr_levels=0, window=0, exp_wait=False, nk_per_chain=None):
+                     corr_p=64, corr_m=8, corr_levels=0, window=0, exp_wait=False, nk_per_chain=None,
+                     partner=None, gex=None):
+        """partner: None/"own" (CD partners have the chain's Nk), "gex" (GEX weight
+        distribution with gex = (a, b, mp, Mk), the reference's PD_flag 1) or "melt" (the
+        ensemble's own chain lengths, weight-averaged)."""
         d = EnsembleDesc()
         d.n_chains, d.nk, d.beta, d.cd = int(n_chains), int(nk), float(beta), int(cd)
         kap = np.zeros(9, np.float32) if kappa is None else np.asarray(kappa, np.float32)
@@ -186,6 +193,10 @@ class Engine:
         d.sample_mode = SAMPLE[sample]
         d.delta = float(delta)
         d.corr_p, d.corr_m, d.corr_levels, d.window = int(corr_p), int(corr_m), int(corr_levels), int(window)
+        d.partner_mode = PARTNER[partner]
+        if gex is not None:
+            for i in range(4):
+                d.gex[i] = float(gex[i])
         keep = None
         if nk_per_chain is not None:
             keep = np.ascontiguousarray(nk_per_chain, np.int32)

@@ -59,7 +59,14 @@ typedef struct {
   int32_t corr_m;               /* multi-tau block factor (reference 8) */
   int32_t corr_levels;          /* levels s; 0 = derive from the run length */
   int32_t window;               /* samples per staging window; 0 = default */
+  int32_t partner_mode;         /* length of the chain across a CD link (constraint-release partner):
+                                   DSM_PARTNER_OWN, DSM_PARTNER_GEX or DSM_PARTNER_MELT */
+  float gex[4];                 /* DSM_PARTNER_GEX: a, b, mp, Mk of polydisp.dat (common/main_cuda.cu:95-103) */
 } dsm_ensemble_desc;
+
+#define DSM_PARTNER_OWN 0     /* the chain's own Nk (reference PD_flag 0) */
+#define DSM_PARTNER_GEX 1     /* GEX weight distribution (reference PD_flag 1, common/gamma.cpp:174-263) */
+#define DSM_PARTNER_MELT 2    /* the ensemble's own chain lengths, weight-averaged (polydisperse melt) */
 
 typedef struct {
   uint64_t events;      /* KMC jumps performed */
@@ -137,7 +144,9 @@ int dsm_correlator(dsm_engine* e, int32_t ens, int32_t max_pts, double* lag, dou
 int dsm_last_samples(dsm_engine* e, int32_t ens, float* out, int64_t max_floats, int64_t* k0, int64_t* nrows);
 
 /* Inspection of the FP32 tables / p_CD constants an ensemble runs with
-   (which: 0 SP 1 SM 2 PP 3 PM 4 EC 5 ED 6 ED2 7 T3 8 CC 9 FR). */
+   (which: 0 SP 1 SM 2 PP 3 PM 4 EC 5 ED 6 ED2 7 T3 8 CC 9 FR; with a polydisperse CD
+   background, 10/11 the f_t / f_d_t sampler constants of the 65536 partner quantiles
+   (5 floats each) and 12/13 the partner Nk of those quantiles). */
 int dsm_tables(dsm_engine* e, int32_t ens, int32_t which, float* out, int32_t* n);
 int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* out24);
 

@@ -243,7 +243,126 @@ int orc_params_init(orc_params* p, float beta, int nk, int nmax, int cd, const f
 void orc_params_free(orc_params* p) {
   free(p->SP); free(p->SM); free(p->PP); free(p->PM); free(p->EC); free(p->ED);
   free(p->ED2); free(p->T3); free(p->CC); free(p->FR);
+  free(p->pf); free(p->pdd); free(p->pf_nk); free(p->pd_nk);
   memset(p, 0, sizeof(*p));
+}
+
+/* ------------------------------------------------------------------ */
+/* Polydisperse CD background                                          */
+/* ------------------------------------------------------------------ */
+/* P(s, x) = gamma(s, x) / Gamma(s): power series for x < s + 1, else 1 - Q with Q from
+   Legendre's continued fraction evaluated by the modified Lentz method. */
+double orc_gamma_p(double s, double x) {
+  if (x <= 0.0) return 0.0;
+  const double lead = -x + s * log(x) - lgamma(s);
+  if (x < s + 1.0) {
+    double term = 1.0 / s, sum = term;
+    for (int n = 1; n < 100000; ++n) {
+      term *= x / (s + n);
+      sum += term;
+      if (fabs(term) < fabs(sum) * 1e-17) break;
+    }
+    return sum * exp(lead);
+  }
+  const double tiny = 1e-300;
+  double b = x + 1.0 - s, cc = 1.0 / tiny, d = 1.0 / b, h = d;
+  for (int i = 1; i < 100000; ++i) {
+    const double an = -i * (i - s);
+    b += 2.0;
+    d = an * d + b;
+    if (fabs(d) < tiny) d = tiny;
+    cc = b + an / cc;
+    if (fabs(cc) < tiny) cc = tiny;
+    d = 1.0 / d;
+    const double del = d * cc;
+    h *= del;
+    if (fabs(del - 1.0) < 1e-16) break;
+  }
+  return 1.0 - exp(lead) * h;
+}
+
+static int cmp_int(const void* a, const void* b) {
+  const int x = *(const int*)a, y = *(const int*)b;
+  return (x > y) - (x < y);
+}
+
+/* Quantile k of the partner weight distribution is the length at probability (k+1/2)/K:
+   mode 1 inverts the GEX weight fraction W(m) = P((a+1)/b, m^b) (common/gamma.cpp:174-231)
+   by bisection and scales Nk = m*mp/Mk; mode 2 weights each distinct chain length by
+   itself times its count.  The f_d_t table takes the same entries re-weighted by their
+   mean destruction rate normdt (common/gamma.cpp:236-262); c_create becomes the mean
+   normdt over the entries divided by beta (common/ensemble.cu:137-146). */
+int orc_params_set_partner(orc_params* p, int mode, const float* gex, const int* nk, int n) {
+  const int K = ORC_PARTNER_K;
+  free(p->pf); free(p->pdd); free(p->pf_nk); free(p->pd_nk);
+  p->pf = calloc(K, sizeof(orc_pcd)); p->pdd = calloc(K, sizeof(orc_pcd));
+  p->pf_nk = calloc(K, 4); p->pd_nk = calloc(K, 4);
+  if (mode == 1) {
+    const double a = gex[0], b = gex[1], mp = gex[2], Mk = gex[3];
+    if (!(a > -1.0) || !(b > 0.0) || !(mp > 0.0) || !(Mk > 0.0)) return -1;
+    const double s = (a + 1.0) / b;
+    for (int k = 0; k < K; ++k) {
+      const double y = (k + 0.5) / K;
+      double lo = 0.0, hi = 1.0;
+      while (orc_gamma_p(s, hi) < y) hi *= 2.0;
+      for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (orc_gamma_p(s, mid) < y) lo = mid; else hi = mid;
+      }
+      const double m = pow(0.5 * (lo + hi), 1.0 / b);
+      p->pf_nk[k] = (float)(m * mp / Mk);
+    }
+  } else if (mode == 2) {
+    int* v = malloc(sizeof(int) * (size_t)n);
+    memcpy(v, nk, sizeof(int) * (size_t)n);
+    qsort(v, (size_t)n, sizeof(int), cmp_int);
+    double tot = 0.0;
+    for (int i = 0; i < n; ++i) tot += (double)v[i];
+    /* distinct values in ascending order with weight value * count */
+    int i = 0;
+    double cum = 0.0;
+    int cur = v[0];
+    while (i < n && v[i] == cur) { cum += (double)v[i]; ++i; }
+    for (int k = 0; k < K; ++k) {
+      const double y = (k + 0.5) / K * tot;
+      while (cum < y && i < n) {
+        cur = v[i];
+        while (i < n && v[i] == cur) { cum += (double)v[i]; ++i; }
+      }
+      p->pf_nk[k] = (float)cur;
+    }
+    free(v);
+  } else {
+    return -1;
+  }
+  double* cum = malloc(sizeof(double) * K);
+  double tot = 0.0;
+  for (int k = 0; k < K; ++k) {
+    orc_pcd_init(&p->pf[k], p->beta, p->pf_nk[k]);
+    tot += (double)p->pf[k].normdt;
+    cum[k] = tot;
+  }
+  int i = 0;
+  for (int j = 0; j < K; ++j) {
+    const double y = (j + 0.5) / K * tot;
+    while (cum[i] < y && i + 1 < K) ++i;
+    p->pdd[j] = p->pf[i];
+    p->pd_nk[j] = p->pf_nk[i];
+  }
+  free(cum);
+  p->partner = mode;
+  p->c_create = (float)(tot / K) / p->beta;
+  for (int N = 1; N < p->nmax + 2; ++N) p->CC[N] = p->cd ? p->c_create * (float)(N - 1) : 0.0f;
+  return 0;
+}
+
+/* sampler constants for a new link: the ensemble's own, or those of the partner quantile
+   in the top bits of Philox word w */
+static const orc_pcd* pcd_ft(const orc_params* p, uint32_t w) {
+  return p->partner ? &p->pf[w >> (32 - ORC_PARTNER_BITS)] : &p->pcd;
+}
+static const orc_pcd* pcd_fdt(const orc_params* p, uint32_t w) {
+  return p->partner ? &p->pdd[w >> (32 - ORC_PARTNER_BITS)] : &p->pcd;
 }
 
 /* ------------------------------------------------------------------ */
@@ -364,13 +483,15 @@ void orc_chain_init_eq(orc_chain* c, const orc_params* p, const uint32_t key[2],
     c->Qx[k] = g0 * sig; c->Qy[k] = g1 * sig; c->Qz[k] = g2 * sig;
   }
   if (p->cd) {
-    uint32_t x[4] = {0, 0, 0, 0};
+    uint32_t x[4] = {0, 0, 0, 0}, y[4] = {0, 0, 0, 0};
     for (int k = 0; k < c->Z - 1; ++k) {
       if ((k & 3) == 0) {
         uint32_t ctr[4] = {(uint32_t)(k >> 2), 0, chain_id, ORC_PURPOSE_INIT_TAU};
         orc_philox4x32_10(ctr, key, x);
+        uint32_t cty[4] = {(uint32_t)(k >> 2), 0, chain_id, ORC_PURPOSE_INIT_PARTNER};
+        if (p->partner) orc_philox4x32_10(cty, key, y);
       }
-      c->w[k] = orc_pcd_rate_ft(&p->pcd, orc_u01(x[k & 3]));
+      c->w[k] = orc_pcd_rate_ft(pcd_ft(p, y[k & 3]), orc_u01(x[k & 3]));
     }
   }
   orc_chain_refresh(c, p);
@@ -532,7 +653,7 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
         NEED_G();
         const int N = c->N[e];
         const int newn = split_newn(r, N, s[3]);
-        const float wnew = orc_pcd_rate_fdt(&p->pcd, u_tau);
+        const float wnew = orc_pcd_rate_fdt(pcd_fdt(p, x[3]), u_tau);
         if (e == 0) {
           const float sig = sqrtf((float)(N - newn) / 3.0f);
           open_gap(c, 0);
@@ -566,7 +687,7 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
         NEED_G();
         const int N = c->N[Z - 1];
         const int newn = split_newn(r, N, endr[0]);
-        const float wnew = orc_pcd_rate_fdt(&p->pcd, u_tau);
+        const float wnew = orc_pcd_rate_fdt(pcd_fdt(p, x[3]), u_tau);
         const float sig = (Z == 1) ? 0.0f : sqrtf((float)newn / 3.0f);
         c->Z = Z + 1;
         if (Z == 1) set_strand(c, Z - 1, newn, 0.0f, 0.0f, 0.0f);
@@ -579,7 +700,7 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
       } else if (m == 1) {   /* SD creation at the head */
         NEED_G();
         const int N0 = c->N[0];
-        const float wnew = orc_pcd_rate_ft(&p->pcd, u_tau);
+        const float wnew = orc_pcd_rate_ft(pcd_ft(p, x[3]), u_tau);
         const float sig = (Z == 1) ? 0.0f : sqrtf((float)(N0 - 1) / 3.0f);
         open_gap(c, 0);
         c->Z = Z + 1;
@@ -595,7 +716,7 @@ int orc_chain_event(orc_chain* c, const orc_params* p, const uint32_t key[2]) {
       } else if (m == 2) {   /* SD creation at the tail */
         NEED_G();
         const int Nz = c->N[Z - 1];
-        const float wnew = orc_pcd_rate_ft(&p->pcd, u_tau);
+        const float wnew = orc_pcd_rate_ft(pcd_ft(p, x[3]), u_tau);
         const float sig = (Z == 1) ? 0.0f : sqrtf((float)(Nz - 1) / 3.0f);
         c->Z = Z + 1;
         if (Z == 1) set_strand(c, Z - 1, Nz - 1, 0.0f, 0.0f, 0.0f);
@@ -786,13 +907,15 @@ void orc_chain64_init_eq(orc_chain64* c, const orc_params* p, const uint32_t key
     c->Qx[k] = g0 * sig; c->Qy[k] = g1 * sig; c->Qz[k] = g2 * sig;
   }
   if (p->cd) {
-    uint32_t x[4] = {0, 0, 0, 0};
+    uint32_t x[4] = {0, 0, 0, 0}, y[4] = {0, 0, 0, 0};
     for (int k = 0; k < c->Z - 1; ++k) {
       if ((k & 3) == 0) {
         uint32_t ctr[4] = {(uint32_t)(k >> 2), 0, chain_id, ORC_PURPOSE_INIT_TAU};
         orc_philox4x32_10(ctr, key, x);
+        uint32_t cty[4] = {(uint32_t)(k >> 2), 0, chain_id, ORC_PURPOSE_INIT_PARTNER};
+        if (p->partner) orc_philox4x32_10(cty, key, y);
       }
-      c->w[k] = orc_pcd_rate_ft_d(&p->pcd, orc_u01_d(x[k & 3]));
+      c->w[k] = orc_pcd_rate_ft_d(pcd_ft(p, y[k & 3]), orc_u01_d(x[k & 3]));
     }
   }
 }
@@ -906,7 +1029,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
     NEED_G64();
     const int N = c->N[e];
     const int newn = split_newn_d(pr, N, wc);
-    const double wnew = orc_pcd_rate_fdt_d(&p->pcd, u_tau);
+    const double wnew = orc_pcd_rate_fdt_d(pcd_fdt(p, x[3]), u_tau);
     for (int i = Z; i > e; --i) mv64(c, i, i - 1);
     c->Z = Z + 1;
     if (e == 0) {
@@ -935,7 +1058,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
       const double sig = (Z == 1) ? 0.0 : sqrt((double)newn / 3.0);
       c->Z = Z + 1;
       set64(c, Z - 1, newn, g[0] * sig, g[1] * sig, g[2] * sig);
-      c->w[Z - 1] = orc_pcd_rate_fdt_d(&p->pcd, u_tau);
+      c->w[Z - 1] = orc_pcd_rate_fdt_d(pcd_fdt(p, x[3]), u_tau);
       set64(c, Z, N - newn, 0, 0, 0);
       c->w[Z] = 0.0;
       kind = ORC_EV_CDC_TAIL;
@@ -946,7 +1069,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
       for (int i = Z; i > 0; --i) mv64(c, i, i - 1);
       c->Z = Z + 1;
       set64(c, 0, 1, 0, 0, 0);
-      c->w[0] = p->cd ? orc_pcd_rate_ft_d(&p->pcd, u_tau) : 0.0;
+      c->w[0] = p->cd ? orc_pcd_rate_ft_d(pcd_ft(p, x[3]), u_tau) : 0.0;
       set64(c, 1, N0 - 1, g[0] * sig, g[1] * sig, g[2] * sig);
       if (p->msd) { c->R1[0] -= c->Qx[1]; c->R1[1] -= c->Qy[1]; c->R1[2] -= c->Qz[1]; }
       kind = ORC_EV_SDC_HEAD;
@@ -956,7 +1079,7 @@ int orc_chain64_event(orc_chain64* c, const orc_params* p, const uint32_t key[2]
       const double sig = (Z == 1) ? 0.0 : sqrt((double)(Nz - 1) / 3.0);
       c->Z = Z + 1;
       set64(c, Z - 1, Nz - 1, g[0] * sig, g[1] * sig, g[2] * sig);
-      c->w[Z - 1] = p->cd ? orc_pcd_rate_ft_d(&p->pcd, u_tau) : 0.0;
+      c->w[Z - 1] = p->cd ? orc_pcd_rate_ft_d(pcd_ft(p, x[3]), u_tau) : 0.0;
       set64(c, Z, 1, 0, 0, 0);
       c->w[Z] = 0.0;
       kind = ORC_EV_SDC_TAIL;
@@ -1249,6 +1372,18 @@ int orc_params_table(const orc_params* p, int which, float* out) {
     case 3: t = p->PM; break; case 4: t = p->EC; break; case 5: t = p->ED; break;
     case 6: t = p->ED2; break; case 7: t = p->T3; break; case 8: t = p->CC; break;
     case 9: t = p->FR; n = p->fr_size; break;
+    case 10: case 11: {
+      if (!p->partner) return 0;
+      const orc_pcd* q = which == 10 ? p->pf : p->pdd;
+      for (int k = 0; k < ORC_PARTNER_K && out; ++k) {
+        float* o = out + 5 * k;
+        if (which == 10) { o[0] = q[k].s_At; o[1] = q[k].s_Ct; o[2] = q[k].s_Dt; o[3] = q[k].s_taud_inv; o[4] = q[k].s_one_minus_g; }
+        else { o[0] = q[k].s_Adt; o[1] = q[k].s_Bdt; o[2] = q[k].s_Cdt; o[3] = q[k].s_Ddt; o[4] = q[k].s_taud_inv; }
+      }
+      return 5 * ORC_PARTNER_K;
+    }
+    case 12: if (!p->partner) return 0; t = p->pf_nk; n = ORC_PARTNER_K; break;
+    case 13: if (!p->partner) return 0; t = p->pd_nk; n = ORC_PARTNER_K; break;
     default: return 0;
   }
   if (out) memcpy(out, t, sizeof(float) * (size_t)n);

@@ -38,6 +38,9 @@ extern "C" {
 #define ORC_PURPOSE_INIT_Q    17u  /* equilibrium strand vectors */
 #define ORC_PURPOSE_INIT_TAU  18u  /* equilibrium CD lifetimes */
 #define ORC_PURPOSE_INIT_NK   19u  /* polydisperse chain length */
+#define ORC_PURPOSE_INIT_PARTNER 20u /* CD partner quantile of each initial link (top 16 bits) */
+#define ORC_PARTNER_BITS 16
+#define ORC_PARTNER_K (1 << ORC_PARTNER_BITS)
 
 /* event kinds (also the GPU's event codes) */
 enum {
@@ -90,6 +93,12 @@ typedef struct {
   float *SP, *SM, *PP, *PM, *EC, *ED, *ED2, *T3, *CC;
   float *FR;       /* size 4*nmax+4: FR[x] = 4/x */
   int fr_size;
+  /* polydisperse CD background (common/gamma.cpp, common/gpu_random.cu:62-76,212-228):
+     ORC_PARTNER_K equiprobable partner-length quantiles for SD-created / initial links
+     (pf) and for CD-created links (pdd, re-weighted by normdt) */
+  int partner;     /* 0 none, 1 GEX, 2 the ensemble's own lengths */
+  orc_pcd *pf, *pdd;
+  float *pf_nk, *pd_nk;
 } orc_params;
 int orc_params_init(orc_params* p, float beta, int nk, int nmax, int cd, const float kappa[9], int exp_wait);
 void orc_params_free(orc_params* p);
@@ -195,9 +204,14 @@ orc_params* orc_params_new(float beta, int nk, int nmax, int cd, const float* ka
 void orc_params_delete(orc_params* p);
 void orc_params_set_flow_samples(orc_params* p, int on);
 void orc_params_set_msd(orc_params* p, int on);
+/* partner mode 1 (gex = a, b, mp, Mk) or 2 (nk[n] chain lengths); rebuilds c_create and
+   the CC table; returns 0 on success */
+int orc_params_set_partner(orc_params* p, int mode, const float* gex, const int* nk, int n);
+double orc_gamma_p(double s, double x);  /* regularised lower incomplete gamma */
 float orc_params_rate_ft(const orc_params* p, float u);
 float orc_params_rate_fdt(const orc_params* p, float u);
-/* which: 0 SP 1 SM 2 PP 3 PM 4 EC 5 ED 6 ED2 7 T3 8 CC 9 FR; returns element count */
+/* which: 0 SP 1 SM 2 PP 3 PM 4 EC 5 ED 6 ED2 7 T3 8 CC 9 FR, 10/11 partner f_t / f_d_t sampler
+   constants (5 per quantile, the device layout), 12/13 partner Nk; returns element count */
 int orc_params_table(const orc_params* p, int which, float* out);
 void orc_params_scalars(const orc_params* p, float* out);  /* 24 floats, see implementation */
 orc_ens* orc_ens_new(const orc_params* p, uint32_t key0, uint32_t key1, uint32_t chain0, int nchains,

@@ -16,7 +16,7 @@ PURPOSE_STEP, PURPOSE_GAUSS = 0, 1
 PURPOSE_INIT_CUT, PURPOSE_INIT_Q, PURPOSE_INIT_TAU, PURPOSE_INIT_NK = 16, 17, 18, 19
 EVENT_NAMES = ["none", "shift+", "shift-", "CDd", "CDc", "CDc_tail", "SDc_head", "SDc_tail",
                "SDd_head", "SDd_tail", "stall"]
-TABLES = ["SP", "SM", "PP", "PM", "EC", "ED", "ED2", "T3", "CC", "FR"]
+TABLES = ["SP", "SM", "PP", "PM", "EC", "ED", "ED2", "T3", "CC", "FR", "PF", "PD", "PF_NK", "PD_NK"]
 SCALARS = ["g", "alpha", "tau_0", "tau_max", "tau_d", "c1", "cm1", "rcm1c1", "At", "Adt", "Bdt",
            "normdt", "s_At", "s_Ct", "s_Dt", "s_Adt", "s_Bdt", "s_Cdt", "s_Ddt", "s_taud_inv",
            "s_one_minus_g", "c_create", "cut_threshold", "beta"]
@@ -54,6 +54,8 @@ def lib():
         L.orc_params_delete.argtypes = [C.c_void_p]
         L.orc_params_set_flow_samples.argtypes = [C.c_void_p, C.c_int]
         L.orc_params_set_msd.argtypes = [C.c_void_p, C.c_int]
+        L.orc_params_set_partner.argtypes = [C.c_void_p, C.c_int, f32p, C.POINTER(C.c_int), C.c_int]
+        L.orc_gamma_p.argtypes = [C.c_double, C.c_double]; L.orc_gamma_p.restype = C.c_double
         L.orc_ens_get_r1.argtypes = [C.c_void_p, f32p]
         L.orc_params_table.argtypes = [C.c_void_p, C.c_int, f32p]; L.orc_params_table.restype = C.c_int
         L.orc_params_scalars.argtypes = [C.c_void_p, f32p]
@@ -153,7 +155,10 @@ def corr_levels(sim_time, delta, p, m):
 class Params:
     """Ensemble parameters and FP32 rate tables (orc_params)."""
 
-    def __init__(self, beta, nk, cd, kappa=None, nmax=None, exp_wait=False):
+    def __init__(self, beta, nk, cd, kappa=None, nmax=None, exp_wait=False, partner=None, gex=None,
+                 nk_chains=None):
+        """partner: None (CD partners have the ensemble's Nk), "gex" (GEX weight distribution,
+        gex = (a, b, mp, Mk)) or "melt" (weight distribution of nk_chains)."""
         self.beta, self.nk, self.cd = float(beta), int(nk), int(cd)
         self.kappa = np.zeros(9, np.float32) if kappa is None else np.asarray(kappa, np.float32)
         self.nmax = int(nmax if nmax is not None else nk)
@@ -161,6 +166,12 @@ class Params:
                                       int(bool(exp_wait)))
         if not self.h:
             raise RuntimeError("orc_params_new failed")
+        if partner is not None:
+            g = np.zeros(4, np.float32) if gex is None else np.asarray(gex, np.float32)
+            nkc = np.ascontiguousarray(nk_chains if nk_chains is not None else [self.nk], np.int32)
+            mode = {"gex": 1, "melt": 2}[partner]
+            if lib().orc_params_set_partner(self.h, mode, _p(g, C.c_float), _p(nkc, C.c_int), nkc.size):
+                raise ValueError("bad partner parameters")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

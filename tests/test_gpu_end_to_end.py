@@ -15,8 +15,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _write_input(path, beta, nk, n, kappa, cd, eq_type, dt, T):
-    lines = [repr(beta), str(nk), str(n), " ".join(repr(float(k)) for k in kappa), str(cd), "0", str(eq_type),
+def _write_input(path, beta, nk, n, kappa, cd, eq_type, dt, T, pd=0):
+    lines = [repr(beta), str(nk), str(n), " ".join(repr(float(k)) for k in kappa), str(cd), str(pd), str(eq_type),
              repr(dt), repr(T)]
     path.write_text("\n".join(lines) + "\n")
 
@@ -99,6 +99,40 @@ def test_cli_gt_run_matches_oracle(tmp_path, oracle, eq_type, name):
     assert 0 < k <= len(lag_o)
     np.testing.assert_allclose(g[:, 0], lag_o[:k])
     np.testing.assert_allclose(g[:, 1], g_o[:k], rtol=2e-5, atol=1e-6 * abs(g_o[0 if eq_type == 1 else 1]))
+
+
+def test_cli_pd_flag_gex_background_matches_oracle(tmp_path, oracle):
+    """PD_flag = 1 reads a, b, mp, Mk from polydisp.dat (common/main_cuda.cu:95-103) and
+    draws CD partners from the GEX background; G(t) and the -s state match the oracle."""
+    beta, nk, n, T, dt, job = 1.0, 30, 96, 60.0, 1.0, 3
+    gex = (1.5, 1.0, 40000.0, 120.0)
+    p, m = 16, 2
+    _write_input(tmp_path / "input.dat", beta, nk, n, [0] * 9, 1, 1, dt, T, pd=1)
+    (tmp_path / "polydisp.dat").write_text(" ".join(repr(x) for x in gex) + "\n")
+    _run(tmp_path, str(job), "--corr-p", str(p), "--corr-m", str(m), "-s", "save.bin", "--quiet")
+    g = np.loadtxt(tmp_path / ("G_%d.dat" % job))
+    s = int(np.ceil(np.log(T / dt / p) / np.log(m))) + 1
+    P = oracle.Params(beta, nk, 1, partner="gex", gex=gex)
+    O = oracle.Ensemble(P, seed=job, n=n, salt=0)
+    _, samples = O.advance(t_end=T, delta=dt, k0=0, k_end=int(T), samples=True)
+    corr = oracle.Correlator(p, m, s, n)
+    for row in samples:
+        corr.add_row(row)
+    lag_o, g_o = corr.result()
+    k = g.shape[0]
+    np.testing.assert_allclose(g[:, 1], g_o[:k], rtol=2e-5, atol=1e-6 * abs(g_o[0]))
+    st = O.state()
+    for i, (Z, _, stall, qn, w) in enumerate(_read_save(tmp_path / "save.bin")):
+        assert Z == st["Z"][i]
+        assert np.array_equal(_bits(w[:Z - 1]), _bits(st["w"][i, :Z - 1]))
+
+
+def test_cli_pd_flag_without_polydisp_file_fails(tmp_path):
+    _write_input(tmp_path / "input.dat", 1.0, 20, 16, [0] * 9, 1, 1, 1.0, 20.0, pd=1)
+    import gpu_dsm_amd as g
+    r = subprocess.run([g.EXE_PATH, "0", "--quiet"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "polydisp.dat" in r.stdout
 
 
 def test_cli_load_rejects_size_mismatch(tmp_path):

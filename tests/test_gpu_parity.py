@@ -149,6 +149,34 @@ def test_kmc_parity(engine, oracle, case, mode):
     engine.remove_ensemble(ens)
 
 
+GEX = (1.5, 1.0, 40000.0, 120.0)  # a, b, mp, Mk
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+@pytest.mark.parametrize("partner", ["gex", "melt"])
+def test_partner_background_parity(engine, oracle, partner, mode):
+    """Polydisperse CD background: partner tables identical to the oracle's, and every
+    CD lifetime drawn through them bit-exact (initial links and created ones), on a
+    polydisperse ensemble whose chains use both the row and the wave form."""
+    rng = np.random.default_rng(7)
+    n = 600
+    nk = rng.choice([12, 30, 60, 140, 400], size=n, p=[0.3, 0.3, 0.2, 0.15, 0.05]).astype(np.int32)
+    nmax = int(nk.max())
+    P = oracle.Params(1.0, nmax, 1, nmax=nmax, partner=partner, gex=GEX, nk_chains=nk)
+    ens = engine.add_ensemble(n, nmax, 1.0, 1, seed=13, salt=2, nk_per_chain=nk, partner=partner, gex=GEX)
+    for name in ("PF", "PD", "PF_NK", "PD_NK", "CC"):
+        assert np.array_equal(_bits(engine.table(ens, name)), _bits(P.table(name))), name
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=13, n=n, salt=2, nk_per_chain=nk)
+    assert_same_state(engine.state(ens), O.state(), "initial")
+    for k, n_ev in enumerate((60, 90)):
+        st = engine.advance_events(ens, n_ev, mode=mode)
+        ev, _ = O.advance(max_events=n_ev)
+        assert st["events"] == ev
+        assert_same_state(engine.state(ens), O.state(), "%s %s chunk %d" % (partner, mode, k))
+    engine.remove_ensemble(ens)
+
+
 def test_batched_time_target_parity(engine, oracle):
     """Run to a time target (persistent form with Z-binning) and compare."""
     P = oracle.Params(1.0, 100, 1)

@@ -59,10 +59,12 @@ def test_driver_reports_missing_input(tmp_path):
     assert rc == -5  # DSM_ERR_IO, the reference would read garbage from a missing input.dat
 
 
-def test_driver_rejects_unsupported_polydisperse_background(tmp_path):
+def test_driver_needs_polydisp_file_for_pd_flag(tmp_path):
+    """PD_flag = 1 reads polydisp.dat before touching the device (common/main_cuda.cu:95-103);
+    without it the driver returns DSM_ERR_IO instead of reading garbage."""
     import gpu_dsm_amd as g
     (tmp_path / "input.dat").write_text("1\n25\n10\n0 0 0 0 0 0 0 0 0\n1\n1\n1\n1.0\n100\n")
-    assert g.run_main(job_id=0, workdir=str(tmp_path)) == -2
+    assert g.run_main(job_id=0, workdir=str(tmp_path)) == -5
 
 
 def test_cli_executable_built():
