@@ -39,7 +39,8 @@ int fail(int code, const char* fmt, ...) {
     if (_e != hipSuccess) return fail(DSM_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(_e)); \
   } while (0)
 
-constexpr int kNumBins = kNumGeoms;  // work bins = KMC chain geometries (capacity 64, 256, 512, 1024)
+constexpr int kNumBins = kNumGeoms;  // work bins = KMC chain geometries (capacity 64, 256, 512, 1024, 2048)
+constexpr int kMaxNk = 2047;         // Z <= Nk must fit the largest geometry after a creation
 constexpr int kDefaultWindow = 256;
 constexpr int kLdsTableFloats = 2816;  // per-wave LDS budget for rate tables (Nk <= ~200)
 
@@ -879,7 +880,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   int nmax = 0;
   for (int i = 0; i < E.n; ++i) {
     const int nk = d->nk_per_chain ? d->nk_per_chain[i] : d->nk;
-    if (nk < 2 || nk > 1023) return fail(DSM_ERR_ARG, "Nk=%d outside 2..1023", nk);
+    if (nk < 2 || nk > kMaxNk) return fail(DSM_ERR_ARG, "Nk=%d outside 2..%d", nk, kMaxNk);
     E.nk[i] = nk;
     nmax = std::max(nmax, nk);
   }
@@ -1323,7 +1324,7 @@ int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, in
   return probe_run(device_id, in, 4 * (size_t)nin, out, 4 * (size_t)nout, l_scan, n, form);
 }
 int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom) {
-  if (!in || !out || geom < 0 || geom >= kNumGeoms || n < 1 || n > kmc_geom_capacity(geom))
+  if (!in || !out || geom < 0 || geom > 3 || n < 1 || n > kmc_geom_capacity(geom))
     return fail(DSM_ERR_ARG, "bad probe arguments");
   return probe_run(device_id, in, 4 * (size_t)(4 * n + 1), out, 4 * 3, l_com, n, geom);
 }

@@ -352,11 +352,13 @@ __global__ void philox_probe_kernel(const uint32_t* ctr_key, uint32_t* out, int 
 namespace dsm {
 
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st) {
+  static_assert(kNumGeoms == 5, "one case per geometry below");
   switch (geom) {
     case 0: return launch_kmc_geom_0(a, plain, grid, st);
     case 1: return launch_kmc_geom_1(a, plain, grid, st);
     case 2: return launch_kmc_geom_2(a, plain, grid, st);
     case 3: return launch_kmc_geom_3(a, plain, grid, st);
+    case 4: return launch_kmc_geom_4(a, plain, grid, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -367,11 +369,12 @@ int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext) {
     case 1: return kmc_occupancy_geom_1(plain, lds_floats, ext);
     case 2: return kmc_occupancy_geom_2(plain, lds_floats, ext);
     case 3: return kmc_occupancy_geom_3(plain, lds_floats, ext);
+    case 4: return kmc_occupancy_geom_4(plain, lds_floats, ext);
     default: return 0;
   }
 }
 
-int kmc_geom_capacity(int geom) { return geom == 0 ? 64 : geom == 1 ? 256 : geom == 2 ? 512 : 1024; }
+int kmc_geom_capacity(int geom) { return geom == 0 ? 64 : 256 << (geom - 1); }
 
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const float* tabs, const int* chains, int n,
                           hipStream_t st) {

@@ -120,7 +120,8 @@ __device__ __forceinline__ int split_newn(float r, int N, float wc) {
 // ------------------------------------------------------------------ geometry
 template <int CPW, int G>
 struct Geo {
-  static_assert((CPW == 4 && G == 1) || (CPW == 1 && G >= 1 && G <= 4), "unsupported chain geometry");
+  static_assert((CPW == 4 && G == 1) || (CPW == 1 && (G == 1 || G == 2 || G == 4 || G == 8)),
+                "unsupported chain geometry");
   static constexpr int R = 4 * G;                        // registers per strand array
   static constexpr int SEG = CPW == 4 ? 16 : 64;         // lanes per chain
   static constexpr int CAP = CPW == 4 ? 64 : 256 * G;    // positions per chain

@@ -1,7 +1,8 @@
 // dsm_kmc_inst.hip -- instantiates the KMC kernels for one chain geometry.
 // Built once per geometry (-DDSM_GEOM=0..3) so the variants compile in parallel:
 //   0: row form, four chains per wave, Z <= 64
-//   1, 2, 3: wave form, one chain per wave, Z <= 256, 512, 1024
+//   1, 2, 3, 4: wave form, one chain per wave, Z <= 256, 512, 1024, 2048 (the last keeps
+//   part of its state in scratch memory; it carries the rare chains with Z > 1024)
 #include "dsm_kmc.h"
 #include "dsm_launch.h"
 
@@ -15,7 +16,11 @@ namespace dsm {
 
 namespace {
 constexpr int kCPW = DSM_GEOM == 0 ? 4 : 1;
-constexpr int kG = DSM_GEOM == 0 ? 1 : (DSM_GEOM == 1 ? 1 : (DSM_GEOM == 2 ? 2 : 4));
+constexpr int kG = DSM_GEOM == 0 ? 1 : (DSM_GEOM == 1 ? 1 : (DSM_GEOM == 2 ? 2 : (DSM_GEOM == 3 ? 4 : 8)));
+// LDS a workgroup may hold: the rate tables are read from global memory when the staging
+// arrays and the table copy would not fit together
+constexpr int kMaxLdsFloats = 65536 / 4;
+bool tables_fit(int lds_floats) { return lds_floats > 0 && stage_floats<kCPW, kG>() + lds_floats <= kMaxLdsFloats; }
 
 template <bool LT, bool PLAIN, bool EXT>
 hipError_t launch_one(const KmcArgs& a, int grid, hipStream_t st) {
@@ -35,15 +40,16 @@ int occupancy_one(int lds_floats) {
 
 template <bool EXT>
 hipError_t launch_variant(const KmcArgs& a, bool plain, int grid, hipStream_t st) {
-  const bool lt = a.lds_floats > 0;
+  const bool lt = tables_fit(a.lds_floats);
   if (plain) return lt ? launch_one<true, true, EXT>(a, grid, st) : launch_one<false, true, EXT>(a, grid, st);
   return lt ? launch_one<true, false, EXT>(a, grid, st) : launch_one<false, false, EXT>(a, grid, st);
 }
 
 template <bool EXT>
 int occupancy_variant(bool plain, int lds_floats) {
-  if (plain) return lds_floats > 0 ? occupancy_one<true, true, EXT>(lds_floats) : occupancy_one<false, true, EXT>(0);
-  return lds_floats > 0 ? occupancy_one<true, false, EXT>(lds_floats) : occupancy_one<false, false, EXT>(0);
+  const bool lt = tables_fit(lds_floats);
+  if (plain) return lt ? occupancy_one<true, true, EXT>(lds_floats) : occupancy_one<false, true, EXT>(0);
+  return lt ? occupancy_one<true, false, EXT>(lds_floats) : occupancy_one<false, false, EXT>(0);
 }
 }  // namespace
 

@@ -43,7 +43,7 @@ double uniform(uint64_t& s) { return (double)(splitmix64(s) >> 11) * 0x1p-53; }
 
 bool valid(const dsm_serve_config* c) {
   return c && c->n_requests > 0 && c->arrival_rate > 0.0 && c->nk_min >= 2 && c->nk_max >= c->nk_min &&
-         c->nk_max <= 1023 && c->nk_step > 0 && c->n_beta >= 1 && c->n_beta <= 4 && c->nch_min >= 1 &&
+         c->nk_max <= 2047 && c->nk_step > 0 && c->n_beta >= 1 && c->n_beta <= 4 && c->nch_min >= 1 &&
          c->nch_max >= c->nch_min && c->flow_fraction >= 0.0 && c->flow_fraction <= 1.0 &&
          (c->flow_fraction == 0.0 || (c->rate_min > 0.0 && c->rate_max >= c->rate_min)) && c->horizon > 0.0 &&
          c->window > 0.0 && c->hbm_fraction > 0.0 && c->hbm_fraction <= 1.0 && c->progress_s >= 0.0;

@@ -37,7 +37,7 @@ def c1(horizon=1.0e5, seed=1):
                 npoints=int(len(g)))
 
 
-def c3_lengths(n=262144, seed=1, nk_lo=10, nk_hi=1023):
+def c3_lengths(n=262144, seed=1, nk_lo=10, nk_hi=2000):
     """Chain lengths of the polydisperse melt: Nk = round(LogNormal(mu, sigma)) with
     number-average 200 and Mw/Mn = 1.5 (sigma^2 = ln 1.5, mu = ln 200 - sigma^2/2),
     clipped to [nk_lo, nk_hi]."""

@@ -196,7 +196,7 @@ typedef struct {
   int32_t n_requests;
   double arrival_rate;       /* requests per second (wall clock) */
   uint64_t seed;             /* request stream; also Philox key word 0 (salt = request index) */
-  int32_t nk_min, nk_max, nk_step;  /* Nk uniform over nk_min, nk_min+step, ..., <= nk_max (<= 1023) */
+  int32_t nk_min, nk_max, nk_step;  /* Nk uniform over nk_min, nk_min+step, ..., <= nk_max (<= 2047) */
   int32_t n_beta;            /* 1..4 */
   float beta[4];             /* beta uniform over the first n_beta entries */
   int32_t nch_min, nch_max;  /* chains per request, log-uniform */

@@ -112,6 +112,7 @@ KMC_CASES = [
     dict(n=96, nk=160, beta=1.0, cd=1, events=250),
     dict(n=48, nk=300, beta=1.0, cd=1, events=200),
     dict(n=9, nk=1000, beta=0.5, cd=1, events=60),
+    dict(n=5, nk=2000, beta=0.5, cd=1, events=30),   # Z ~ 1330: the Z <= 2048 wave geometry
     # thousands of work units: several per persistent wave, and chains that reach the
     # row form's capacity (Z = 64) mid-pass and continue in the wave form
     dict(n=4096, nk=100, beta=1.0, cd=1, events=80),

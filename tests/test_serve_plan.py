@@ -37,7 +37,7 @@ def test_memory_estimate_grows_with_the_request():
     assert np.all(eq["bytes"] > 0)
 
 
-@pytest.mark.parametrize("bad", [dict(n_requests=0), dict(nk_max=2000), dict(nk_step=0), dict(arrival_rate=0.0),
+@pytest.mark.parametrize("bad", [dict(n_requests=0), dict(nk_max=2048), dict(nk_step=0), dict(arrival_rate=0.0),
                                  dict(beta=()), dict(hbm_fraction=1.5), dict(horizon=0.0)])
 def test_invalid_streams_are_rejected(bad):
     import gpu_dsm_amd as g
