@@ -41,3 +41,36 @@ def test_gt_error_against_fp64_oracle(engine, oracle, case):
     print("G(t) FP32 device vs FP64 oracle, %s: max |dG|/G(0) = %.3g over %d lags" % (
         "nk%d cd%d" % (nk, cd), err, have.size))
     assert err < 0.02
+
+
+@pytest.mark.parametrize("rate", [1e-3, 1e-2, 1e-1])
+def test_shear_stress_error_against_fp64_oracle(engine, oracle, rate):
+    """Steady-shear observables of the sweep (shear stress and first normal stress
+    difference, time-averaged over the second half of the run): the device's FP32 flow
+    path against the FP64 oracle on the same chains and Philox streams.
+    Bar: |d sigma_xy| and |d N1| below 2% of the FP64 value's magnitude (+ a floor of
+    2% of G(0)-scale stress, <Z>, for rates where the signal is small)."""
+    n, nk, T, dlt = 256, 50, 100.0, 1.0
+    kappa = [0, rate, 0, 0, 0, 0, 0, 0, 0]
+    ens = engine.add_ensemble(n, nk, 1.0, 1, kappa=kappa, seed=31, sample="flow", delta=dlt, window=32)
+    engine.init_equilibrium(ens)
+    engine.advance(ens, T)
+    rows = engine.flow_stress(ens)
+    engine.remove_ensemble(ens)
+    P = oracle.Params(1.0, nk, 1, kappa=kappa)
+    O = oracle.Ensemble(P, seed=31, n=n, fp64=True)
+    k_end = int(T / dlt) + 1
+    _, s = O.advance(t_end=T, delta=dlt, k0=1, k_end=k_end, samples=True, flow_samples=True)
+    ref = s.sum(axis=1)                       # [row][8] sums over chains, FP64
+    half = slice(rows.shape[0] // 2, None)
+    sxy_g = np.mean(rows[half, 3] / rows[half, 8])
+    n1_g = np.mean((rows[half, 0] - rows[half, 1]) / rows[half, 8])
+    sxy_o = np.mean(ref[half, 3] / n)
+    n1_o = np.mean((ref[half, 0] - ref[half, 1]) / n)
+    zbar = np.mean(ref[half, 7] / n)
+    e_sxy = abs(sxy_g - sxy_o) / max(abs(sxy_o), zbar)
+    e_n1 = abs(n1_g - n1_o) / max(abs(n1_o), zbar)
+    print("shear %g: sigma_xy %.5g (FP64 %.5g), N1 %.5g (FP64 %.5g), rel err %.2g / %.2g" % (
+        rate, sxy_g, sxy_o, n1_g, n1_o, e_sxy, e_n1))
+    assert np.all(rows[:, 8] == n)
+    assert e_sxy < 0.02 and e_n1 < 0.02
