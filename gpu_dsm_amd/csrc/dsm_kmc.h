@@ -1130,6 +1130,143 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg,
   if (H.c >= 0 && H.ovf && L.sl == 0) a.overflow[atomicAdd(a.n_overflow, 1)] = H.c;
 }
 
+// Batched form with per-segment refill.  Each segment of the wave (a 16-lane row in the
+// row form, the whole wave in the wave form) takes chains one at a time from the work
+// list: when its chain reaches the window end (or overflows the geometry) the chain is
+// written back and the segment takes the next one, while the other segments keep
+// jumping.  A quad of chains thus no longer waits for its slowest member.  A wave runs
+// chains of one ensemble at a time: a segment that takes a chain of another ensemble
+// holds it until the wave's running chains have finished, then the wave switches.
+// Every (re)load restages the whole wave and recomputes every pair rate from the staged
+// strands, which reproduces the cached rates of the running chains exactly, and refills
+// the random-number cache from each chain's own jump count (the cache holds the Philox
+// outputs of jumps ev .. ev+SEG-1 of each chain), so a chain's trajectory does not
+// depend on which chains share its wave.
+template <int CPW, int G, bool LT, bool EXT>
+__device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* ltab) {
+  constexpr int CAP = Geo<CPW, G>::CAP;
+  constexpr int SEG = Geo<CPW, G>::SEG;
+  // A segment's state lives in H.c and H.inert (no extra registers):
+  //   H.c == -1: empty;  H.c == -2: past the end of the list;
+  //   H.c >= 0 and H.inert == kHeld: chain taken, not loaded (another ensemble);
+  //   H.inert == 0: running;  1: overflowed while running (written back, handed on);
+  //   kDeadAtLoad: stalled or over capacity at load (not written back).
+  constexpr int kHeld = 2, kDeadAtLoad = 3;
+  const LaneC<CPW> L;
+  const int n_work = *a.n_work;
+  int lds_ens = -1, cur = -1;
+  EnsLocal E;
+  Tabs<LT> T;
+  Hdr H;
+  H.c = -1; H.Z = 1; H.nk = 1; H.stall = 0; H.local = 0; H.cid = 0; H.next_sample = 0; H.t = 0.0;
+  H.t_next = 0.0; H.ev = 0; H.zsum = 0; H.dt_prev = 0.0f; H.left = 0; H.inert = 1; H.e1 = 0.0f;
+  H.ovf = 0; H.done = 0; H.dlo = 1 << 30; H.dhi = -1;
+  Stage<CPW, G> S{stg, L.srow};
+  Strands<G> C;
+#pragma unroll
+  for (int j = 0; j < 4 * G; ++j) { C.qx[j] = 0.f; C.qy[j] = 0.f; C.qz[j] = 0.f; C.q2[j] = 0.f; C.w[j] = 0.f; C.n[j] = 0; }
+  RandCache RC{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int k_rng = SEG;
+  for (;;) {
+    // 1. empty segments take entries until they hold a chain or the list is exhausted
+    for (;;) {
+      const bool need = H.c == -1;
+      if (!__ballot(need)) break;
+      int ent = 0;
+      if (need && L.sl == 0) ent = atomicAdd(a.queue, 1);
+      ent = seg_get<CPW>(ent, 0, L);
+      if (need) {
+        const int c = ent < n_work ? a.work[ent] : -2;
+        if (c != -1) { H.c = c; H.inert = kHeld; }
+      }
+    }
+    // 2. the ensemble: switch when nothing of the current one runs or waits
+    const bool held = H.c >= 0 && H.inert == kHeld;
+    const int pe = held ? a.ch.ens[H.c] : -1;
+    if (!__ballot(H.c >= 0 && (H.inert != kHeld || pe == cur))) {
+      const uint64_t hm = __ballot(held);
+      if (!hm) break;  // every segment is past the end of the list
+      cur = __builtin_amdgcn_readlane(pe, (int)__builtin_ctzll(hm));
+      load_ens(E, a.ens, cur);
+      T.n = E.nmax + 2;
+      if constexpr (LT) {
+        if (lds_ens != cur) {
+          const int tsz = 9 * T.n + 4 * E.nmax + 4;
+          const float* src = a.tabs + E.P->tab_off;
+          for (int i = L.lane; i < tsz; i += 64) ltab[i] = src[i];
+          wave_lds_sync();
+          lds_ens = cur;
+        }
+        T.b = ltab;
+      } else {
+        T.b = (typename TabPtr<false>::type)(a.tabs + E.P->tab_off);
+      }
+    }
+    // 3. load the held chains of the current ensemble, restage the wave
+    const bool ld = held && pe == cur;
+    if (__ballot(ld)) {
+      if (ld) {
+        const ChainArrays& ch = a.ch;
+        const int c = H.c;
+        H.Z = ch.Z[c]; H.nk = ch.nk[c]; H.stall = ch.stall[c]; H.local = ch.local[c]; H.cid = ch.cid[c];
+        H.next_sample = ch.next_sample[c]; H.t = ch.t[c]; H.ev = ch.ev[c];
+        H.zsum = ch.zsum[c]; H.dt_prev = ch.dt_prev[c];
+        const uint64_t stop = a.use_ev_stop ? ch.ev_stop[c] : ~0ull;
+        H.left = stop <= H.ev ? 0 : (stop - H.ev > 0x7fffffffull ? 0x7fffffff : (int)(stop - H.ev));
+        const bool dead = H.stall != 0 || H.Z > CAP;
+        H.ovf = (H.stall == 0 && H.Z > CAP) ? 1 : 0;
+        H.inert = dead ? kDeadAtLoad : 0;
+        if (dead) H.Z = 1;
+        H.t_next = next_time(E, H.next_sample);
+        H.e1 = 1.0f / (E.beta * (float)H.nk);
+        load_strands<CPW, G>(C, a.ch, H, a.ch.base[c], L);
+      }
+      stage_n<CPW, G>(S, C, L);
+      stage_qw<CPW, G>(S, C, L);
+      refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+      wave_lds_sync();
+      // the cache is refilled at every chain's current jump count before the next jump
+      // (clearing it here tells the compiler the old values are dead in this branch)
+      RC = RandCache{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      k_rng = SEG;
+    }
+    // 4. jumps until a loaded chain finishes
+    bool fin = false;
+    for (;;) {
+      if (E.flow) {
+        const bool dfm = !H.inert && H.dt_prev != 0.0f;
+        if (__ballot(dfm)) {
+          if (dfm) deform<CPW, G>(C, E.P, H.dt_prev, H.Z, L);
+          stage_qw<CPW, G>(S, C, L);
+          refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+          wave_lds_sync();
+        }
+        H.dt_prev = 0.0f;
+      }
+      record_samples<CPW, G, LT, EXT>(C, T, E, H, a.stage, a.ch.R1, L);
+      bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0;
+      if (act && H.Z >= CAP) { H.ovf = 1; act = false; }
+      if (H.ovf && !H.inert) H.inert = 1;
+      fin = H.c >= 0 && H.inert != kHeld && !act;
+      if (__ballot(fin)) break;
+      jump<CPW, G, LT, EXT>(C, H, act, S, T, E, RC, k_rng, a.ch.R1, a.tabs, L);
+    }
+    // 5. write back the finished chains and hand on the overflowed ones
+    if (fin) {
+      if (H.inert != kDeadAtLoad) {
+        store_strands<CPW, G>(C, a.ch, H, a.ch.base[H.c], 0, H.Z - 1, L);
+        if (L.sl == 0) {
+          const int c = H.c;
+          a.ch.Z[c] = H.Z; a.ch.stall[c] = H.stall; a.ch.next_sample[c] = H.next_sample; a.ch.t[c] = H.t;
+          a.ch.ev[c] = H.ev; a.ch.zsum[c] = H.zsum; a.ch.dt_prev[c] = H.dt_prev;
+        }
+      }
+      if (H.ovf && L.sl == 0) a.overflow[atomicAdd(a.n_overflow, 1)] = H.c;
+      H.c = -1; H.inert = 1; H.ovf = 0;
+    }
+  }
+}
+
 // LDS floats per wave: staging arrays, then the table copy
 template <int CPW, int G>
 __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G>::STG; }
@@ -1140,18 +1277,22 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
 // centre-of-mass samples) and the polydisperse CD background (partner tables).
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
-__global__ void __launch_bounds__(64) kmc_kernel(KmcArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((EXT || PLAIN || CPW != 4) ? 1 : 3))) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<CPW, G>();
-  int lds_ens = -1;
-  const int n_units = (*a.n_work + CPW - 1) / CPW;
-  for (;;) {
-    int unit = 0;
-    if (lane_id() == 0) unit = atomicAdd(a.queue, 1);
-    unit = __builtin_amdgcn_readfirstlane(unit);
-    if (unit >= n_units) break;
-    run_unit<CPW, G, LT, PLAIN, EXT>(a, unit, stg, ltab, lds_ens);
+  if constexpr (!PLAIN && CPW == 4) {
+    run_stream<CPW, G, LT, EXT>(a, stg, ltab);
+  } else {
+    int lds_ens = -1;
+    const int n_units = (*a.n_work + CPW - 1) / CPW;
+    for (;;) {
+      int unit = 0;
+      if (lane_id() == 0) unit = atomicAdd(a.queue, 1);
+      unit = __builtin_amdgcn_readfirstlane(unit);
+      if (unit >= n_units) break;
+      run_unit<CPW, G, LT, PLAIN, EXT>(a, unit, stg, ltab, lds_ens);
+    }
   }
 }
 
