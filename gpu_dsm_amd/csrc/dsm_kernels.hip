@@ -93,7 +93,10 @@ __global__ void __launch_bounds__(1024) sum_counters_kernel(const uint64_t* ev, 
                                                              int n, unsigned long long* out) {
   __shared__ unsigned long long r0[1024], r1[1024];
   unsigned long long a0 = 0, a1 = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) { a0 += ev[chain0 + i]; a1 += zsum[chain0 + i]; }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    a0 += ev[chain0 + i];
+    a1 += zsum[chain0 + i];
+  }
   r0[threadIdx.x] = a0;
   r1[threadIdx.x] = a1;
   __syncthreads();
@@ -101,7 +104,7 @@ __global__ void __launch_bounds__(1024) sum_counters_kernel(const uint64_t* ev, 
     if ((int)threadIdx.x < h) { r0[threadIdx.x] += r0[threadIdx.x + h]; r1[threadIdx.x] += r1[threadIdx.x + h]; }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { out[0] += r0[0]; out[1] += r1[0]; }
+  if (threadIdx.x == 0) { atomicAdd(out, r0[0]); atomicAdd(out + 1, r1[0]); }
 }
 
 __global__ void set_ev_stop_kernel(const uint64_t* ev, uint64_t* ev_stop, int chain0, int n, uint64_t add) {
@@ -170,60 +173,144 @@ struct CorrArgs {
   int type;             // 0: stress (block averages, products); 1: MSD (first value, squared differences)
 };
 
+// One value into level `lvl` and, through the block pushes, the levels above it
+// (D, A and C in device memory).
+__device__ __forceinline__ void corr_insert_from(const CorrArgs& a, int c, int lvl, double vx, double vy, double vz,
+                                                 int* ins, int* macc, int* nins) {
+  const int64_t nch = a.nch;
+  const int p = a.p, m = a.m;
+  for (;;) {
+    if (lvl == a.s) break;
+    const int in = ins[lvl];
+    double* D = a.D + (int64_t)(lvl * p) * 3 * nch + c;
+    D[(int64_t)(in * 3 + 0) * nch] = vx;
+    D[(int64_t)(in * 3 + 1) * nch] = vy;
+    D[(int64_t)(in * 3 + 2) * nch] = vz;
+    double* A = a.A + (int64_t)(lvl * 3) * nch + c;
+    // MSD carries the first value of each block up unaveraged
+    // (common/eq_ensemble_kernel.cu:181-189)
+    if (a.type == 0 || macc[lvl] == 0) { A[0] += vx; A[nch] += vy; A[2 * nch] += vz; }
+    macc[lvl]++;
+    const bool push = macc[lvl] == m;
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    if (push) {
+      if (a.type == 0) { ux = A[0] / m; uy = A[nch] / m; uz = A[2 * nch] / m; }
+      else { ux = A[0]; uy = A[nch]; uz = A[2 * nch]; }
+      A[0] = 0.0; A[nch] = 0.0; A[2 * nch] = 0.0;
+      macc[lvl] = 0;
+    }
+    const int jmin = (lvl == 0) ? 0 : p / m;
+    double* Cc = a.C + (int64_t)(lvl * p) * nch + c;
+    for (int j = jmin; j < p; ++j) {
+      if (j > nins[lvl]) break;
+      int idx = in - j;
+      if (idx < 0) idx += p;
+      const double dx = D[(int64_t)(idx * 3 + 0) * nch];
+      const double dy = D[(int64_t)(idx * 3 + 1) * nch];
+      const double dz = D[(int64_t)(idx * 3 + 2) * nch];
+      if (a.type == 0) {
+        Cc[(int64_t)j * nch] += vx * dx + vy * dy + vz * dz;
+      } else {
+        const double ex = vx - dx, ey = vy - dy, ez = vz - dz;
+        Cc[(int64_t)j * nch] += ex * ex + ey * ey + ez * ez;
+      }
+    }
+    ins[lvl] = (in + 1 == p) ? 0 : in + 1;
+    nins[lvl] = nins[lvl] < p ? nins[lvl] + 1 : p;
+    if (!push) break;
+    ++lvl;
+    vx = ux; vy = uy; vz = uz;
+  }
+}
+
+// Any p: every level in device memory.
 __global__ void __launch_bounds__(256) corr_update_kernel(CorrArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.nch) return;
   int ins[40], macc[40], nins[40];
   for (int k = 0; k < a.s; ++k) { ins[k] = a.cnt[k]; macc[k] = a.cnt[a.s + k]; nins[k] = a.cnt[2 * a.s + k]; }
   const int64_t nch = a.nch;
-  const int p = a.p, m = a.m;
   for (int row = 0; row < a.nrows; ++row) {
     const float* o = a.stage + ((int64_t)row * nch + c) * 3;
-    double vx = o[0], vy = o[1], vz = o[2];
-    int lvl = 0;
-    for (;;) {
-      if (lvl == a.s) break;
-      const int in = ins[lvl];
-      double* D = a.D + (int64_t)(lvl * p) * 3 * nch + c;
-      D[(int64_t)(in * 3 + 0) * nch] = vx;
-      D[(int64_t)(in * 3 + 1) * nch] = vy;
-      D[(int64_t)(in * 3 + 2) * nch] = vz;
-      double* A = a.A + (int64_t)(lvl * 3) * nch + c;
-      // MSD carries the first value of each block up unaveraged
-      // (common/eq_ensemble_kernel.cu:181-189)
-      if (a.type == 0 || macc[lvl] == 0) { A[0] += vx; A[nch] += vy; A[2 * nch] += vz; }
-      macc[lvl]++;
-      const bool push = macc[lvl] == m;
-      double ux = 0.0, uy = 0.0, uz = 0.0;
-      if (push) {
-        if (a.type == 0) { ux = A[0] / m; uy = A[nch] / m; uz = A[2 * nch] / m; }
-        else { ux = A[0]; uy = A[nch]; uz = A[2 * nch]; }
-        A[0] = 0.0; A[nch] = 0.0; A[2 * nch] = 0.0;
-        macc[lvl] = 0;
-      }
-      const int jmin = (lvl == 0) ? 0 : p / m;
-      double* Cc = a.C + (int64_t)(lvl * p) * nch + c;
-      for (int j = jmin; j < p; ++j) {
-        if (j > nins[lvl]) break;
-        int idx = in - j;
-        if (idx < 0) idx += p;
-        const double dx = D[(int64_t)(idx * 3 + 0) * nch];
-        const double dy = D[(int64_t)(idx * 3 + 1) * nch];
-        const double dz = D[(int64_t)(idx * 3 + 2) * nch];
-        if (a.type == 0) {
-          Cc[(int64_t)j * nch] += vx * dx + vy * dy + vz * dz;
-        } else {
-          const double ex = vx - dx, ey = vy - dy, ez = vz - dz;
-          Cc[(int64_t)j * nch] += ex * ex + ey * ey + ez * ez;
-        }
-      }
-      ins[lvl] = (in + 1 == p) ? 0 : in + 1;
-      nins[lvl] = nins[lvl] < p ? nins[lvl] + 1 : p;
-      if (!push) break;
-      ++lvl;
-      vx = ux; vy = uy; vz = uz;
-    }
+    corr_insert_from(a, c, 0, o[0], o[1], o[2], ins, macc, nins);
   }
+}
+
+// p = P (16 in every reference input): level 0, which takes every sample row, is held
+// in registers for the whole window -- its P slots in age order (h[0] the newest), its P
+// lag sums and its block accumulator -- and written back once; the levels above take
+// the block pushes in device memory as before.  Every sum is formed in the same order
+// as in corr_update_kernel, so both give identical results.
+template <int P>
+__global__ void __launch_bounds__(256) corr_update_reg_kernel(CorrArgs a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nch) return;
+  int ins[40], macc[40], nins[40];
+  for (int k = 1; k < a.s; ++k) { ins[k] = a.cnt[k]; macc[k] = a.cnt[a.s + k]; nins[k] = a.cnt[2 * a.s + k]; }
+  int in0 = a.cnt[0], macc0 = a.cnt[a.s], nins0 = a.cnt[2 * a.s];
+  const int64_t nch = a.nch;
+  const int m = a.m;
+  double* D = a.D + c;
+  double* A = a.A + c;
+  double* Cc = a.C + c;
+  double hx[P], hy[P], hz[P], cs[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int slot = (in0 - 1 - k + 2 * P) % P;
+    hx[k] = D[(int64_t)(slot * 3 + 0) * nch];
+    hy[k] = D[(int64_t)(slot * 3 + 1) * nch];
+    hz[k] = D[(int64_t)(slot * 3 + 2) * nch];
+    cs[k] = Cc[(int64_t)k * nch];
+  }
+  double ax = A[0], ay = A[nch], az = A[2 * nch];
+  for (int row = 0; row < a.nrows; ++row) {
+    const float* o = a.stage + ((int64_t)row * nch + c) * 3;
+    const double vx = o[0], vy = o[1], vz = o[2];
+    if (a.type == 0 || macc0 == 0) { ax += vx; ay += vy; az += vz; }
+    ++macc0;
+    const bool push = macc0 == m;
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    if (push) {
+      if (a.type == 0) { ux = ax / m; uy = ay / m; uz = az / m; }
+      else { ux = ax; uy = ay; uz = az; }
+      ax = 0.0; ay = 0.0; az = 0.0;
+      macc0 = 0;
+    }
+    // lag j pairs the new value with the one inserted j steps before it (h[j-1])
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const double dx = j == 0 ? vx : hx[j - 1 < 0 ? 0 : j - 1];
+      const double dy = j == 0 ? vy : hy[j - 1 < 0 ? 0 : j - 1];
+      const double dz = j == 0 ? vz : hz[j - 1 < 0 ? 0 : j - 1];
+      double add;
+      if (a.type == 0) {
+        add = vx * dx + vy * dy + vz * dz;
+      } else {
+        const double ex = vx - dx, ey = vy - dy, ez = vz - dz;
+        add = ex * ex + ey * ey + ez * ez;
+      }
+      if (j <= nins0) cs[j] += add;
+    }
+#pragma unroll
+    for (int k = P - 1; k > 0; --k) { hx[k] = hx[k - 1]; hy[k] = hy[k - 1]; hz[k] = hz[k - 1]; }
+    hx[0] = vx; hy[0] = vy; hz[0] = vz;
+    in0 = (in0 + 1 == P) ? 0 : in0 + 1;
+    nins0 = nins0 < P ? nins0 + 1 : P;
+    if (push) corr_insert_from(a, c, 1, ux, uy, uz, ins, macc, nins);
+  }
+  // write back: the slots of this window's values (the newest min(nrows, P)), the lag
+  // sums and the block accumulator
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    if (k < a.nrows) {
+      const int slot = (in0 - 1 - k + 2 * P) % P;
+      D[(int64_t)(slot * 3 + 0) * nch] = hx[k];
+      D[(int64_t)(slot * 3 + 1) * nch] = hy[k];
+      D[(int64_t)(slot * 3 + 2) * nch] = hz[k];
+    }
+    Cc[(int64_t)k * nch] = cs[k];
+  }
+  A[0] = ax; A[nch] = ay; A[2 * nch] = az;
 }
 
 // G(level, lag) = sum over healthy chains of C / (3 * n_pairs * n_healthy)
@@ -399,7 +486,8 @@ hipError_t launch_set_ev_stop(const uint64_t* ev, uint64_t* ev_stop, int chain0,
 hipError_t launch_sum_counters(const uint64_t* ev, const uint64_t* zsum, int chain0, int n, unsigned long long* out,
                                hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sum_counters_kernel, dim3(1), dim3(1024), 0, st, ev, zsum, chain0, n, out);
+  const int grid = n > 256 * 4096 ? 256 : (n + 4095) / 4096;  // integer sums: the order does not matter
+  hipLaunchKernelGGL(sum_counters_kernel, dim3(grid), dim3(1024), 0, st, ev, zsum, chain0, n, out);
   return hipGetLastError();
 }
 
@@ -420,7 +508,10 @@ hipError_t launch_corr_update(const float* stage, double* D, double* A, double* 
                               int nch, int p, int m, int s, int nrows, const int* cnt, int type, hipStream_t st) {
   if (nrows <= 0 || nch <= 0) return hipSuccess;
   CorrArgs a{stage, D, A, C, stall, chain0, nch, p, m, s, nrows, cnt, type};
-  hipLaunchKernelGGL(corr_update_kernel, dim3((nch + 255) / 256), dim3(256), 0, st, a);
+  if (p == 16)
+    hipLaunchKernelGGL(corr_update_reg_kernel<16>, dim3((nch + 255) / 256), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(corr_update_kernel, dim3((nch + 255) / 256), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -329,3 +329,27 @@ def test_multi_ensemble_flow_batch(engine, oracle, mode):
         np.testing.assert_allclose(rows[:, :8], samples.astype(np.float64).sum(axis=1), rtol=1e-12, atol=1e-9)
     for ens in ids:
         engine.remove_ensemble(ens)
+
+
+def test_many_small_ensembles_batch(engine, oracle):
+    """Many small ensembles of different lengths in one batched pass (single chains,
+    counts that are not multiples of four, chains in the row and the wave form): a
+    16-lane segment that takes a chain of another ensemble holds it until its wave has
+    switched ensembles, so every chain must still match its own oracle run bit for bit."""
+    counts = [1, 2, 1, 7, 33, 1, 130, 3, 64, 1]
+    nks = [20, 90, 150, 50, 100, 8, 70, 300, 40, 60]
+    T = 20.0
+    ids, refs = [], []
+    for q, (n, nk) in enumerate(zip(counts, nks)):
+        P = oracle.Params(1.0, nk, 1)
+        ens = engine.add_ensemble(n, nk, 1.0, 1, seed=60 + q, salt=q)
+        engine.init_equilibrium(ens)
+        O = oracle.Ensemble(P, seed=60 + q, n=n, salt=q)
+        O.advance(t_end=T)
+        ids.append(ens)
+        refs.append(O)
+    engine.advance_many(ids, T)
+    for ens, O in zip(ids, refs):
+        assert_same_state(engine.state(ens), O.state(), "ensemble %d" % ens)
+    for ens in ids:
+        engine.remove_ensemble(ens)
