@@ -40,6 +40,12 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 constexpr int kNumBins = kNumGeoms;  // work bins = KMC chain geometries (capacity 64, 256, 512, 1024, 2048)
+// Work lists (cap entries each).  Sequential pass: list b feeds bin b, overflow to b+1.
+// Concurrent pass: list 0 holds the pass's chains, lists 1..kNumBins the per-geometry
+// split (round 1), lists kNumBins+1.. the chains that outgrew their geometry in a round
+// (round 2: late list of bin b = kNumBins + b).
+constexpr int kNumLists = 2 * kNumBins + 1;
+constexpr int kSplitList = 1, kLateList = kNumBins;
 constexpr int kMaxNk = 2047;         // Z <= Nk must fit the largest geometry after a creation
 constexpr int kDefaultWindow = 256;
 constexpr int kLdsTableFloats = 2816;  // per-wave LDS budget for rate tables (Nk <= ~200)
@@ -315,6 +321,10 @@ struct Ens {
 struct dsm_engine {
   int device = 0;
   hipStream_t st = nullptr;
+  hipStream_t xs[kNumBins] = {};  // xs[b], b >= 1: geometry b of a concurrent pass (default priority)
+  hipEvent_t ev_split = nullptr, ev_bin[kNumBins] = {};
+  hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};  // concurrent pass: start, row start/end, end
+  hipEvent_t lev[2 * kNumBins] = {};                        // sequential pass: each launch's start/end
   int n_cu = 0;
   // chains
   int nch = 0;
@@ -341,7 +351,7 @@ struct dsm_engine {
   // work lists: [bin][nch] + counts
   int* d_work = nullptr;
   int d_work_cap = 0;
-  int* d_cnt = nullptr;     // [kNumBins + 1] list counts, then [kNumBins + 1] queues
+  int* d_cnt = nullptr;     // [kNumLists] list counts, then [kNumLists] queues
   unsigned long long* d_counters = nullptr;
   int* d_corr_cnt = nullptr;
   int d_corr_cnt_cap = 0;
@@ -423,7 +433,7 @@ int grow_chains(dsm_engine* e, int n_need, int64_t s_need) {
     e->d_work = nullptr;
     e->d_work_cap = 0;
     const int cap = need + need / 8 + 1024;
-    if ((rc = dev_alloc(&e->d_work, (size_t)(kNumBins + 1) * cap))) return rc;
+    if ((rc = dev_alloc(&e->d_work, (size_t)kNumLists * cap))) return rc;
     e->d_work_cap = cap;
   }
   return DSM_OK;
@@ -586,29 +596,88 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   return DSM_OK;
 }
 
-// One pass through the geometry bins.  Batched: each chain runs to its targets;
-// plain: each chain makes one jump.  Times every launch when `timed`.
-int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs) {
+// Batched pass over chains that may need the larger geometries (a polydisperse melt, a
+// mix of ensembles): the chains are split by their current Z into one list per
+// geometry and every geometry runs at once, each on its own stream, the largest first
+// (its few long chains take their slots before the bulk fills the device).  The long
+// chains' latency-bound runs then overlap the bulk instead of following it.  Chains that
+// outgrow their geometry during this round go to a late list that a second, sequential
+// round runs through the larger geometries.  Every chain still makes the same jumps to
+// the same targets, so the results are those of the sequential order.
+int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   int* cnt = e->d_cnt;
-  HIPCHK(launch_fill_u32(cnt, 0u, 2 * (kNumBins + 1), e->st));
+  const size_t cap = (size_t)e->d_work_cap;
+  auto list = [&](int l) { return e->d_work + (size_t)l * cap; };
+  KmcArgs a = ps.a;
+  auto bind = [&](int in, int out) {
+    a.work = list(in);
+    a.n_work = cnt + in;
+    a.queue = cnt + kNumLists + in;
+    a.overflow = list(out);
+    a.n_overflow = cnt + out;
+  };
+  auto grid = [&](int b) { return device_blocks(e, b, false, a.lds_floats, a.ext != 0); };
+  HIPCHK(hipEventRecord(e->cev[0], e->st));
+  HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumLists, e->st));
+  HIPCHK(launch_split_geoms(list(0), ps.n0, e->ch.Z, e->ch.stall, list(kSplitList), (int)cap, cnt + kSplitList,
+                            e->st));
+  HIPCHK(hipEventRecord(e->ev_split, e->st));
+  // round 1
+  for (int b = ps.bin_max; b >= 1; --b) {
+    HIPCHK(hipStreamWaitEvent(e->xs[b], e->ev_split, 0));
+    bind(kSplitList + b, kLateList + b + 1);
+    HIPCHK(launch_kmc(a, b, false, grid(b), e->xs[b]));
+    HIPCHK(hipEventRecord(e->ev_bin[b], e->xs[b]));
+  }
+  bind(kSplitList, kLateList + 1);
+  HIPCHK(hipEventRecord(e->cev[1], e->st));
+  HIPCHK(launch_kmc(a, 0, false, grid(0), e->st));
+  HIPCHK(hipEventRecord(e->cev[2], e->st));
+  for (int b = 1; b <= ps.bin_max; ++b) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[b], 0));
+  // round 2: chains that outgrew their geometry, through the larger ones in order
+  for (int b = 1; b <= ps.bin_max; ++b) {
+    bind(kLateList + b, kLateList + b + 1);
+    HIPCHK(launch_kmc(a, b, false, grid(b), e->st));
+  }
+  HIPCHK(hipEventRecord(e->cev[3], e->st));
+  HIPCHK(hipEventSynchronize(e->cev[3]));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e->cev[0], e->cev[3]));
+  rs->kmc_ms += ms;
+  HIPCHK(hipEventElapsedTime(&ms, e->cev[1], e->cev[2]));
+  rs->first_kmc_ms += ms;
+  rs->kmc_launches += 1 + 2 * ps.bin_max;
+  rs->first_launches++;
+  return DSM_OK;
+}
+
+// One pass through the geometry bins in order (list b feeds bin b, overflow to b+1).
+// Batched: each chain runs to its targets; plain: each chain makes one jump.  Times
+// every launch when `timed` (one host wait at the end).
+int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs) {
+  if (!plain && ps.bin_max >= 2) return pass_bins_concurrent(e, ps, rs);
+  int* cnt = e->d_cnt;
+  HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumLists, e->st));
   HIPCHK(launch_fill_u32(cnt, (uint32_t)ps.n0, 1, e->st));
   KmcArgs a = ps.a;
   for (int b = 0; b <= ps.bin_max; ++b) {
     a.work = e->d_work + (size_t)b * e->d_work_cap;
     a.n_work = cnt + b;
-    a.queue = cnt + (kNumBins + 1) + b;
+    a.queue = cnt + kNumLists + b;
     a.overflow = e->d_work + (size_t)(b + 1) * e->d_work_cap;
     a.n_overflow = cnt + b + 1;
     const int grid = device_blocks(e, b, plain, a.lds_floats, a.ext != 0);
-    if (timed) HIPCHK(hipEventRecord(e->evs[0], e->st));
+    if (timed) HIPCHK(hipEventRecord(e->lev[2 * b], e->st));
     HIPCHK(launch_kmc(a, b, plain, grid, e->st));
+    if (timed) HIPCHK(hipEventRecord(e->lev[2 * b + 1], e->st));
     rs->kmc_launches++;
     if (b == 0) rs->first_launches++;
-    if (timed) {
-      HIPCHK(hipEventRecord(e->evs[1], e->st));
-      HIPCHK(hipEventSynchronize(e->evs[1]));
+  }
+  if (timed) {
+    HIPCHK(hipEventSynchronize(e->lev[2 * ps.bin_max + 1]));
+    for (int b = 0; b <= ps.bin_max; ++b) {
       float ms = 0.0f;
-      HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+      HIPCHK(hipEventElapsedTime(&ms, e->lev[2 * b], e->lev[2 * b + 1]));
       rs->kmc_ms += ms;
       if (b == 0) rs->first_kmc_ms += ms;
     }
@@ -821,13 +890,20 @@ int dsm_create(int32_t device_id, dsm_engine** out) {
   dsm_engine* e = new dsm_engine();
   e->device = device_id;
   e->n_cu = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
-    delete e;
+  bool streams_ok = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) == hipSuccess;
+  for (int b = 1; b < kNumBins; ++b)
+    streams_ok = streams_ok && hipStreamCreateWithFlags(&e->xs[b], hipStreamNonBlocking) == hipSuccess;
+  if (!streams_ok) {
+    dsm_destroy(e);
     return fail(DSM_ERR_DEVICE, "stream creation failed");
   }
   for (auto& ev : e->evs) HIPCHK(hipEventCreate(&ev));
+  for (auto& ev : e->cev) HIPCHK(hipEventCreate(&ev));
+  for (auto& ev : e->lev) HIPCHK(hipEventCreate(&ev));
+  HIPCHK(hipEventCreateWithFlags(&e->ev_split, hipEventDisableTiming));
+  for (auto& ev : e->ev_bin) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   int rc;
-  if ((rc = dev_alloc(&e->d_cnt, 4 * (kNumBins + 1))) || (rc = dev_alloc(&e->d_counters, 4)) ||
+  if ((rc = dev_alloc(&e->d_cnt, 2 * kNumLists)) || (rc = dev_alloc(&e->d_counters, 4)) ||
       (rc = dev_alloc(&e->d_healthy, 1))) {
     dsm_destroy(e);
     return rc;
@@ -846,6 +922,8 @@ int dsm_destroy(dsm_engine* e) {
   if (!e) return DSM_OK;
   (void)hipSetDevice(e->device);
   if (e->st) (void)hipStreamSynchronize(e->st);
+  for (auto& x : e->xs)
+    if (x) (void)hipStreamSynchronize(x);
   for (auto& E : e->ens) {
     if (E.dD) (void)hipFree(E.dD);
     if (E.dA) (void)hipFree(E.dA);
@@ -859,6 +937,15 @@ int dsm_destroy(dsm_engine* e) {
     if (p) (void)hipFree(p);
   for (auto& ev : e->evs)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : e->cev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : e->lev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : e->ev_bin)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->ev_split) (void)hipEventDestroy(e->ev_split);
+  for (auto& x : e->xs)
+    if (x) (void)hipStreamDestroy(x);
   if (e->st) (void)hipStreamDestroy(e->st);
   delete e;
   return DSM_OK;

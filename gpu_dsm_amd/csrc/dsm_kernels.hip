@@ -20,6 +20,7 @@
 //              registers for many jumps (until its clock passes the window end),
 //              then write it back once.
 #include "dsm_kmc.h"
+#include "dsm_launch.h"
 
 namespace dsm {
 
@@ -110,6 +111,22 @@ __global__ void __launch_bounds__(1024) sum_counters_kernel(const uint64_t* ev, 
 __global__ void set_ev_stop_kernel(const uint64_t* ev, uint64_t* ev_stop, int chain0, int n, uint64_t add) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) ev_stop[chain0 + i] = ev[chain0 + i] + add;
+}
+
+// Chains of a pass into per-geometry lists by their current Z: geometry b holds a chain
+// that can still jump in it (Z below its capacity); stalled chains go to geometry 0,
+// which skips them.  Lists are cap entries apart, counts[b] their lengths.
+__global__ void split_geoms_kernel(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
+                                   int* counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = src[i];
+  if (c < 0) return;
+  const int z = Z[c];
+  int b = 0;
+  if (stall[c] == 0)
+    while (b + 1 < kNumGeoms && z >= (b == 0 ? 64 : 256 << (b - 1))) ++b;
+  lists[(int64_t)b * cap + atomicAdd(counts + b, 1)] = c;
 }
 
 __global__ void iota_kernel(int* p, int start, int n) {
@@ -488,6 +505,14 @@ hipError_t launch_sum_counters(const uint64_t* ev, const uint64_t* zsum, int cha
   if (n <= 0) return hipSuccess;
   const int grid = n > 256 * 4096 ? 256 : (n + 4095) / 4096;  // integer sums: the order does not matter
   hipLaunchKernelGGL(sum_counters_kernel, dim3(grid), dim3(1024), 0, st, ev, zsum, chain0, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_geoms(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
+                              int* counts, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(split_geoms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, src, n, Z, stall, lists, cap,
+                     counts);
   return hipGetLastError();
 }
 
