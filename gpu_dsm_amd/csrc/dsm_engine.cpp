@@ -39,7 +39,7 @@ int fail(int code, const char* fmt, ...) {
     if (_e != hipSuccess) return fail(DSM_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(_e)); \
   } while (0)
 
-constexpr int kNumBins = kNumGeoms;  // work bins = KMC chain geometries (capacity 64, 256, 512, 1024, 2048)
+constexpr int kNumBins = kNumGeoms;  // work bins = KMC chain geometries (capacity 64 << b)
 // Work lists (cap entries each).  Sequential pass: list b feeds bin b, overflow to b+1.
 // Concurrent pass: list 0 holds the pass's chains, lists 1..kNumBins the per-geometry
 // split (round 1), lists kNumBins+1.. the chains that outgrew their geometry in a round
@@ -299,6 +299,7 @@ struct Ens {
   int64_t strand0 = 0, nstrand = 0;
   EnsParams P{};
   int bin_max = 0;      // largest geometry bin a chain of this ensemble can need
+  double z_sum_est = 0.0;  // sum over its chains of the equilibrium mean Z, (Nk + beta) / (1 + beta)
   // sampling
   int ncomp = 0;        // floats per sample per chain (3 G(t), 8 flow)
   int window = 0;
@@ -568,12 +569,16 @@ int chain_totals(dsm_engine* e, const std::vector<int>& ids, unsigned long long 
 struct Pass {
   int n0 = 0;  // entries of the bin-0 list
   int bin_max = 0;
+  bool concurrent = false;  // batched passes only: split by Z, geometries side by side
   KmcArgs a{};
 };
 
 int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pass& ps) {
   int n0 = 0, bin_max = 0, tab_max = 0, ext = 0;
+  double z_est = 0.0, n_all = 0.0;
   for (int id : ids) {
+    z_est += e->ens[id].z_sum_est;
+    n_all += e->ens[id].n;
     ext |= (e->ens[id].P.sample_mode == kSampleMsd || e->ens[id].P.partner) ? 1 : 0;
     n0 += (e->ens[id].n + 3) & ~3;
     bin_max = std::max(bin_max, e->ens[id].bin_max);
@@ -581,6 +586,10 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   }
   ps.n0 = n0;
   ps.bin_max = bin_max;
+  // geometries side by side when chains beyond the first wave-form geometry can occur
+  // (their runs are long and few) or when many chains are expected above the row form's
+  // 63 positions (the pair form then carries the bulk)
+  ps.concurrent = bin_max >= 3 || (bin_max >= 1 && z_est >= 56.0 * n_all);
   if (n0 == 0) return DSM_OK;
   HIPCHK(launch_fill_u32(e->d_work, 0xFFFFFFFFu, n0, e->st));
   int off = 0;
@@ -596,14 +605,19 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   return DSM_OK;
 }
 
-// Batched pass over chains that may need the larger geometries (a polydisperse melt, a
-// mix of ensembles): the chains are split by their current Z into one list per
-// geometry and every geometry runs at once, each on its own stream, the largest first
-// (its few long chains take their slots before the bulk fills the device).  The long
-// chains' latency-bound runs then overlap the bulk instead of following it.  Chains that
-// outgrow their geometry during this round go to a late list that a second, sequential
-// round runs through the larger geometries.  Every chain still makes the same jumps to
-// the same targets, so the results are those of the sequential order.
+// Chains that outgrow the row (or pair) form during a pass continue in the wave form
+// (geometry 2, Z <= 256): these few chains form the pass's tail, and the wave form has
+// the lowest per-jump latency.  Last geometry a pass runs them through:
+constexpr int kLateFirst = 2;
+int late_last(int bin_max) { return std::max(bin_max, kLateFirst); }
+
+// Batched pass with the geometries side by side: the chains are split by their current
+// Z into one list per geometry and every geometry runs at once, each on its own
+// stream, the largest first (its few long chains take their slots before the bulk fills
+// the device), so the latency-bound runs of long chains overlap the bulk instead of
+// following it.  Chains that outgrow their geometry during this round go to late lists
+// that a second, sequential round runs through the larger geometries.  Every chain makes
+// the same jumps to the same targets as in the sequential order.
 int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   int* cnt = e->d_cnt;
   const size_t cap = (size_t)e->d_work_cap;
@@ -617,6 +631,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
     a.n_overflow = cnt + out;
   };
   auto grid = [&](int b) { return device_blocks(e, b, false, a.lds_floats, a.ext != 0); };
+  auto late_of = [&](int b) { return kLateList + (b < kLateFirst ? kLateFirst : b + 1); };  // bin b's overflow
   HIPCHK(hipEventRecord(e->cev[0], e->st));
   HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumLists, e->st));
   HIPCHK(launch_split_geoms(list(0), ps.n0, e->ch.Z, e->ch.stall, list(kSplitList), (int)cap, cnt + kSplitList,
@@ -625,18 +640,18 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // round 1
   for (int b = ps.bin_max; b >= 1; --b) {
     HIPCHK(hipStreamWaitEvent(e->xs[b], e->ev_split, 0));
-    bind(kSplitList + b, kLateList + b + 1);
+    bind(kSplitList + b, late_of(b));
     HIPCHK(launch_kmc(a, b, false, grid(b), e->xs[b]));
     HIPCHK(hipEventRecord(e->ev_bin[b], e->xs[b]));
   }
-  bind(kSplitList, kLateList + 1);
+  bind(kSplitList, late_of(0));
   HIPCHK(hipEventRecord(e->cev[1], e->st));
   HIPCHK(launch_kmc(a, 0, false, grid(0), e->st));
   HIPCHK(hipEventRecord(e->cev[2], e->st));
   for (int b = 1; b <= ps.bin_max; ++b) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[b], 0));
   // round 2: chains that outgrew their geometry, through the larger ones in order
-  for (int b = 1; b <= ps.bin_max; ++b) {
-    bind(kLateList + b, kLateList + b + 1);
+  for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) {
+    bind(kLateList + b, late_of(b));
     HIPCHK(launch_kmc(a, b, false, grid(b), e->st));
   }
   HIPCHK(hipEventRecord(e->cev[3], e->st));
@@ -646,40 +661,46 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   rs->kmc_ms += ms;
   HIPCHK(hipEventElapsedTime(&ms, e->cev[1], e->cev[2]));
   rs->first_kmc_ms += ms;
-  rs->kmc_launches += 1 + 2 * ps.bin_max;
+  rs->kmc_launches += 1 + ps.bin_max + (late_last(ps.bin_max) - kLateFirst + 1);
   rs->first_launches++;
   return DSM_OK;
 }
 
-// One pass through the geometry bins in order (list b feeds bin b, overflow to b+1).
-// Batched: each chain runs to its targets; plain: each chain makes one jump.  Times
-// every launch when `timed` (one host wait at the end).
+// One pass through the geometry bins in order: the row form, then (for chains that
+// outgrow it) the wave form and the larger geometries; list b feeds bin b.  Batched:
+// each chain runs to its targets; plain: each chain makes one jump.  Times every launch
+// when `timed` (one host wait at the end).
 int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs) {
-  if (!plain && ps.bin_max >= 2) return pass_bins_concurrent(e, ps, rs);
+  if (!plain && ps.concurrent) return pass_bins_concurrent(e, ps, rs);
   int* cnt = e->d_cnt;
   HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumLists, e->st));
   HIPCHK(launch_fill_u32(cnt, (uint32_t)ps.n0, 1, e->st));
   KmcArgs a = ps.a;
-  for (int b = 0; b <= ps.bin_max; ++b) {
+  int seq[kNumBins], ns = 0;
+  seq[ns++] = 0;
+  if (ps.bin_max >= 1)
+    for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) seq[ns++] = b;
+  for (int k = 0; k < ns; ++k) {
+    const int b = seq[k], nb = k + 1 < ns ? seq[k + 1] : b + 1;
     a.work = e->d_work + (size_t)b * e->d_work_cap;
     a.n_work = cnt + b;
     a.queue = cnt + kNumLists + b;
-    a.overflow = e->d_work + (size_t)(b + 1) * e->d_work_cap;
-    a.n_overflow = cnt + b + 1;
+    a.overflow = e->d_work + (size_t)nb * e->d_work_cap;
+    a.n_overflow = cnt + nb;
     const int grid = device_blocks(e, b, plain, a.lds_floats, a.ext != 0);
-    if (timed) HIPCHK(hipEventRecord(e->lev[2 * b], e->st));
+    if (timed) HIPCHK(hipEventRecord(e->lev[2 * k], e->st));
     HIPCHK(launch_kmc(a, b, plain, grid, e->st));
-    if (timed) HIPCHK(hipEventRecord(e->lev[2 * b + 1], e->st));
+    if (timed) HIPCHK(hipEventRecord(e->lev[2 * k + 1], e->st));
     rs->kmc_launches++;
     if (b == 0) rs->first_launches++;
   }
   if (timed) {
-    HIPCHK(hipEventSynchronize(e->lev[2 * ps.bin_max + 1]));
-    for (int b = 0; b <= ps.bin_max; ++b) {
+    HIPCHK(hipEventSynchronize(e->lev[2 * ns - 1]));
+    for (int k = 0; k < ns; ++k) {
       float ms = 0.0f;
-      HIPCHK(hipEventElapsedTime(&ms, e->lev[2 * b], e->lev[2 * b + 1]));
+      HIPCHK(hipEventElapsedTime(&ms, e->lev[2 * k], e->lev[2 * k + 1]));
       rs->kmc_ms += ms;
-      if (b == 0) rs->first_kmc_ms += ms;
+      if (k == 0) rs->first_kmc_ms += ms;
     }
   }
   return DSM_OK;
@@ -970,6 +991,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
     if (nk < 2 || nk > kMaxNk) return fail(DSM_ERR_ARG, "Nk=%d outside 2..%d", nk, kMaxNk);
     E.nk[i] = nk;
     nmax = std::max(nmax, nk);
+    E.z_sum_est += ((double)nk + d->beta) / (1.0 + d->beta);
   }
   E.nmax = nmax;
   E.bin_max = bin_for(nmax);
@@ -1405,13 +1427,14 @@ static hipError_t l_philox(const void* i, void* o, int n, int, hipStream_t s) {
 }
 
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form) {
-  if (!in || !out || (form != 0 && form != 1) || n <= 0 || n > (form == 1 ? 1024 : 64))
+  if (!in || !out || form < 0 || form > 2 || n <= 0 || n > (form == 1 ? 1024 : form == 2 ? 128 : 64))
     return fail(DSM_ERR_ARG, "bad probe arguments");
-  const int nin = form == 1 ? n : 4 * n, nout = form == 1 ? n + 1 : 4 * (n + 1);
+  const int chains = form == 1 ? 1 : form == 2 ? 2 : 4;
+  const int nin = chains * n, nout = chains * (n + 1);
   return probe_run(device_id, in, 4 * (size_t)nin, out, 4 * (size_t)nout, l_scan, n, form);
 }
 int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom) {
-  if (!in || !out || geom < 0 || geom > 3 || n < 1 || n > kmc_geom_capacity(geom))
+  if (!in || !out || geom < 0 || geom > 4 || n < 1 || n > kmc_geom_capacity(geom))
     return fail(DSM_ERR_ARG, "bad probe arguments");
   return probe_run(device_id, in, 4 * (size_t)(4 * n + 1), out, 4 * 3, l_com, n, geom);
 }

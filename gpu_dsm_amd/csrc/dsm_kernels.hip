@@ -125,7 +125,7 @@ __global__ void split_geoms_kernel(const int* src, int n, const int* Z, const in
   const int z = Z[c];
   int b = 0;
   if (stall[c] == 0)
-    while (b + 1 < kNumGeoms && z >= (b == 0 ? 64 : 256 << (b - 1))) ++b;
+    while (b + 1 < kNumGeoms && z >= kmc_geom_capacity(b)) ++b;
   lists[(int64_t)b * cap + atomicAdd(counts + b, 1)] = c;
 }
 
@@ -363,10 +363,11 @@ __global__ void copy_probe_kernel(const float4* __restrict__ in, float4* __restr
 // ------------------------------------------------------------------ scan probe (tests)
 // Canonical scan tree of the KMC kernels.  form 1: one chain of n <= 1024 positions
 // (wave form, four register groups): out[0..n) inclusive sums, out[n] total.
-// form 0: four chains of n <= 64 positions (row form): chain r reads in[r*n ..] and
-// writes out[r*(n+1) ..].
-// Centre-of-mass sample of one chain state in a chosen geometry (0 row form, 1/2/3 wave
-// form G = 1/2/4): in[4i..4i+3] = Qx Qy Qz N of strand i, in[4n] = Nk; out = 3 floats.
+// form 0: four chains of n <= 64 positions (row form), form 2: two chains of n <= 128
+// positions (pair form): chain r reads in[r*n ..] and writes out[r*(n+1) ..].
+// Centre-of-mass sample of one chain state in a chosen geometry (0 row form, 1 pair form,
+// 2/3/4 wave form G = 1/2/4): in[4i..4i+3] = Qx Qy Qz N of strand i, in[4n] = Nk;
+// out = 3 floats.
 template <int CPW, int G>
 __device__ void com_probe(const float* in, float* out, int n) {
   const LaneC<CPW> L;
@@ -388,8 +389,9 @@ __device__ void com_probe(const float* in, float* out, int n) {
 }
 __global__ void __launch_bounds__(64) com_probe_kernel(const float* in, float* out, int n, int geom) {
   if (geom == 0) com_probe<4, 1>(in, out, n);
-  else if (geom == 1) com_probe<1, 1>(in, out, n);
-  else if (geom == 2) com_probe<1, 2>(in, out, n);
+  else if (geom == 1) com_probe<2, 1>(in, out, n);
+  else if (geom == 2) com_probe<1, 1>(in, out, n);
+  else if (geom == 3) com_probe<1, 2>(in, out, n);
   else com_probe<1, 4>(in, out, n);
 }
 
@@ -409,6 +411,22 @@ __global__ void __launch_bounds__(64) scan_probe_kernel(const float* in, float* 
       if (p < n) out[p] = g[j];
     }
     if (L.lane == 0) out[n] = tot;
+  } else if (form == 2) {
+    const LaneC<2> L;
+    const int r = L.lane >> 5;
+    float v[4], g[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = L.pos(j);
+      v[j] = p < n ? in[r * n + p] : 0.0f;
+    }
+    const float tot = chain_scan<2, 1>(v, g, n, L);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = L.pos(j);
+      if (p < n) out[r * (n + 1) + p] = g[j];
+    }
+    if (L.sl == 0) out[r * (n + 1) + n] = tot;
   } else {
     const LaneC<4> L;
     const int r = L.lane >> 4;
@@ -456,13 +474,14 @@ __global__ void philox_probe_kernel(const uint32_t* ctr_key, uint32_t* out, int 
 namespace dsm {
 
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st) {
-  static_assert(kNumGeoms == 5, "one case per geometry below");
+  static_assert(kNumGeoms == 6, "one case per geometry below");
   switch (geom) {
     case 0: return launch_kmc_geom_0(a, plain, grid, st);
     case 1: return launch_kmc_geom_1(a, plain, grid, st);
     case 2: return launch_kmc_geom_2(a, plain, grid, st);
     case 3: return launch_kmc_geom_3(a, plain, grid, st);
     case 4: return launch_kmc_geom_4(a, plain, grid, st);
+    case 5: return launch_kmc_geom_5(a, plain, grid, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -474,11 +493,11 @@ int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext) {
     case 2: return kmc_occupancy_geom_2(plain, lds_floats, ext);
     case 3: return kmc_occupancy_geom_3(plain, lds_floats, ext);
     case 4: return kmc_occupancy_geom_4(plain, lds_floats, ext);
+    case 5: return kmc_occupancy_geom_5(plain, lds_floats, ext);
     default: return 0;
   }
 }
 
-int kmc_geom_capacity(int geom) { return geom == 0 ? 64 : 256 << (geom - 1); }
 
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const float* tabs, const int* chains, int n,
                           hipStream_t st) {

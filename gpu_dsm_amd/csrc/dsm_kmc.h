@@ -120,12 +120,12 @@ __device__ __forceinline__ int split_newn(float r, int N, float wc) {
 // ------------------------------------------------------------------ geometry
 template <int CPW, int G>
 struct Geo {
-  static_assert((CPW == 4 && G == 1) || (CPW == 1 && (G == 1 || G == 2 || G == 4 || G == 8)),
+  static_assert(((CPW == 4 || CPW == 2) && G == 1) || (CPW == 1 && (G == 1 || G == 2 || G == 4 || G == 8)),
                 "unsupported chain geometry");
   static constexpr int R = 4 * G;                        // registers per strand array
-  static constexpr int SEG = CPW == 4 ? 16 : 64;         // lanes per chain
-  static constexpr int CAP = CPW == 4 ? 64 : 256 * G;    // positions per chain
-  static constexpr int STG = CPW == 4 ? 256 : 256 * G;   // floats per staging array
+  static constexpr int SEG = 64 / CPW;                   // lanes per chain
+  static constexpr int CAP = 4 * SEG * G;                // positions per chain
+  static constexpr int STG = 256 * G;                    // floats per staging array (all chains of a wave)
 };
 
 // LDS staging arrays of a wave (each STG floats; chain positions at srow + p)
@@ -140,10 +140,10 @@ struct LaneC {
   int srow;  // staging offset of the chain's position 0
   __device__ __forceinline__ LaneC() {
     lane = lane_id();
-    sl = CPW == 4 ? (lane & 15) : lane;
-    sb = CPW == 4 ? (lane & 48) : 0;
+    sl = lane & (64 / CPW - 1);
+    sb = lane & ~(64 / CPW - 1) & 63;
     pos0 = 4 * sl;
-    srow = CPW == 4 ? 64 * (lane >> 4) : 0;
+    srow = 4 * sb;
   }
   __device__ __forceinline__ int pos(int j) const { return 256 * (j >> 2) + pos0 + (j & 3); }
 };
@@ -173,6 +173,24 @@ template <> __device__ __forceinline__ float seg_next<4, float>(float v) { retur
 template <> __device__ __forceinline__ float seg_next<1, float>(float v) { return dppf<kWaveShl1>(v, 0.0f); }
 template <> __device__ __forceinline__ int seg_next<4, int>(int v) { return dppi<kRowShl1>(v, 0); }
 template <> __device__ __forceinline__ int seg_next<1, int>(int v) { return dppi<kWaveShl1>(v, 0); }
+// two chains per wave (32-lane segments): the wave shift crosses rows inside a segment;
+// the lane at the segment's start (end) takes 0 instead of its neighbour's value
+template <> __device__ __forceinline__ float seg_prev<2, float>(float v) {
+  const float x = dppf<kWaveShr1>(v, 0.0f);
+  return (lane_id() & 31) == 0 ? 0.0f : x;
+}
+template <> __device__ __forceinline__ int seg_prev<2, int>(int v) {
+  const int x = dppi<kWaveShr1>(v, 0);
+  return (lane_id() & 31) == 0 ? 0 : x;
+}
+template <> __device__ __forceinline__ float seg_next<2, float>(float v) {
+  const float x = dppf<kWaveShl1>(v, 0.0f);
+  return (lane_id() & 31) == 31 ? 0.0f : x;
+}
+template <> __device__ __forceinline__ int seg_next<2, int>(int v) {
+  const int x = dppi<kWaveShl1>(v, 0);
+  return (lane_id() & 31) == 31 ? 0 : x;
+}
 
 __device__ __forceinline__ float rdl(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -239,6 +257,17 @@ __device__ __forceinline__ float chain_scan(const float (&v)[4 * G], float (&out
   if constexpr (CPW == 4) {
     block_scan4(v[0], v[1], v[2], v[3], out);
     return row_last(out[3]);
+  } else if constexpr (CPW == 2) {
+    // blocks 0 and 1 of chain s are rows 2s and 2s+1; block 1 adds block 0's total
+    float I[4];
+    block_scan4(v[0], v[1], v[2], v[3], I);
+    const float b00 = rdl(I[3], 15), b01 = rdl(I[3], 31), b10 = rdl(I[3], 47), b11 = rdl(I[3], 63);
+    const bool s1 = L.lane >= 32;
+    const float B0 = s1 ? b10 : b00, B1 = s1 ? b11 : b01;
+    const bool upper = (L.lane & 16) != 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = upper ? B0 + I[k] : I[k];
+    return Z > 64 ? B0 + B1 : B0;
   } else {
     const int nb = ((__builtin_amdgcn_readfirstlane(Z) - 1) >> 6) + 1;
     const int row = L.lane >> 4;
@@ -284,6 +313,17 @@ __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr,
     cand = min(cand, dppi<kRowShr4>(cand, kIdent));
     cand = min(cand, dppi<kRowShr8>(cand, kIdent));
     return row_last(cand);
+  } else if constexpr (CPW == 2) {
+    int cand = CAP;
+#pragma unroll
+    for (int k = 3; k >= 0; --k) cand = (Gv[k] > pr && L.pos0 + k < Z) ? L.pos0 + k : cand;
+    constexpr int kIdent = 0x7fffffff;
+    cand = min(cand, dppi<kRowShr1>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr2>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr4>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr8>(cand, kIdent));
+    const int r0 = rdl(cand, 15), r1 = rdl(cand, 31), r2 = rdl(cand, 47), r3 = rdl(cand, 63);
+    return L.lane >= 32 ? min(r2, r3) : min(r0, r1);
   } else {
     int res = CAP;
 #pragma unroll
@@ -1051,6 +1091,7 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg,
   const LaneC<CPW> L;
   const int c0 = __builtin_amdgcn_readfirstlane(a.work[CPW * unit]);
   const int ens = __builtin_amdgcn_readfirstlane(a.ch.ens[c0]);
+  const int n_entries = *a.n_work;  // lists of the larger geometries are not padded
   EnsLocal E;
   load_ens(E, a.ens, ens);
   Tabs<LT> T;
@@ -1068,7 +1109,10 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg,
     T.b = (typename TabPtr<false>::type)(a.tabs + E.P->tab_off);
   }
   Hdr H;
-  H.c = (CPW == 4) ? a.work[CPW * unit + (L.lane >> 4)] : c0;
+  {
+    const int ent = CPW * unit + L.lane / Geo<CPW, G>::SEG;
+    H.c = CPW == 1 ? c0 : (ent < n_entries ? a.work[ent] : -1);
+  }
   H.ovf = 0; H.done = 0; H.dlo = 1 << 30; H.dhi = -1;
   if (H.c >= 0) {
     const ChainArrays& ch = a.ch;
@@ -1277,11 +1321,11 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
 // centre-of-mass samples) and the polydisperse CD background (partner tables).
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((EXT || PLAIN || CPW != 4) ? 1 : 3))) kmc_kernel(KmcArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((EXT || PLAIN || CPW == 1) ? 1 : 3))) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<CPW, G>();
-  if constexpr (!PLAIN && CPW == 4) {
+  if constexpr (!PLAIN && CPW > 1) {
     run_stream<CPW, G, LT, EXT>(a, stg, ltab);
   } else {
     int lds_ens = -1;

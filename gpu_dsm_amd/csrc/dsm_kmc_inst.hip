@@ -1,7 +1,8 @@
 // dsm_kmc_inst.hip -- instantiates the KMC kernels for one chain geometry.
-// Built once per geometry (-DDSM_GEOM=0..3) so the variants compile in parallel:
+// Built once per geometry (-DDSM_GEOM=0..5) so the variants compile in parallel:
 //   0: row form, four chains per wave, Z <= 64
-//   1, 2, 3, 4: wave form, one chain per wave, Z <= 256, 512, 1024, 2048 (the last keeps
+//   1: pair form, two chains per wave, Z <= 128
+//   2, 3, 4, 5: wave form, one chain per wave, Z <= 256, 512, 1024, 2048 (the last keeps
 //   part of its state in scratch memory; it carries the rare chains with Z > 1024)
 #include "dsm_kmc.h"
 #include "dsm_launch.h"
@@ -15,8 +16,9 @@
 namespace dsm {
 
 namespace {
-constexpr int kCPW = DSM_GEOM == 0 ? 4 : 1;
-constexpr int kG = DSM_GEOM == 0 ? 1 : (DSM_GEOM == 1 ? 1 : (DSM_GEOM == 2 ? 2 : (DSM_GEOM == 3 ? 4 : 8)));
+constexpr int kCPW = DSM_GEOM == 0 ? 4 : (DSM_GEOM == 1 ? 2 : 1);
+constexpr int kG = DSM_GEOM <= 2 ? 1 : (DSM_GEOM == 3 ? 2 : (DSM_GEOM == 4 ? 4 : 8));
+static_assert(Geo<kCPW, kG>::CAP == (64 << DSM_GEOM), "geometry g holds 64 << g positions");
 // LDS a workgroup may hold: the rate tables are read from global memory when the staging
 // arrays and the table copy would not fit together
 constexpr int kMaxLdsFloats = 65536 / 4;

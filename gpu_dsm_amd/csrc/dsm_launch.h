@@ -8,14 +8,15 @@
 namespace dsm {
 
 // KMC kernels by chain geometry: 0 = row form (four chains per wave, Z <= 64),
-// 1/2/3/4 = wave form (one chain per wave, Z <= 256/512/1024/2048).  Persistent: grid =
+// 1 = pair form (two chains per wave, Z <= 128), 2/3/4/5 = wave form (one chain per
+// wave, Z <= 256/512/1024/2048): geometry g holds 64 << g positions per chain.  Persistent: grid =
 // blocks of one wave; plain = one jump per chain.  lds_floats > 0 stages the rate
 // tables (that many floats) in LDS.  a.ext selects the variant with MSD(t) tracking and
 // polydisperse CD partners.
-constexpr int kNumGeoms = 5;
+constexpr int kNumGeoms = 6;
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st);
 int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext);
-int kmc_geom_capacity(int geom);  // positions per chain
+__host__ __device__ inline int kmc_geom_capacity(int geom) { return 64 << geom; }  // positions per chain
 #define DSM_DECLARE_GEOM(G)                                                                   \
   hipError_t launch_kmc_geom_##G(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
   int kmc_occupancy_geom_##G(bool plain, int lds_floats, bool ext);
@@ -24,6 +25,7 @@ DSM_DECLARE_GEOM(1)
 DSM_DECLARE_GEOM(2)
 DSM_DECLARE_GEOM(3)
 DSM_DECLARE_GEOM(4)
+DSM_DECLARE_GEOM(5)
 #undef DSM_DECLARE_GEOM
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const float* tabs, const int* chains, int n,
                           hipStream_t st);

@@ -320,7 +320,8 @@ class Engine:
 # ---------------------------------------------------------------- building-block probes
 def probe_com(Q, N, geom, device=0):
     """MSD centre-of-mass sample (relative to R1) of one chain state computed by the
-    KMC kernels' code in geometry `geom` (0 row form, 1/2/3 wave form G = 1/2/4)."""
+    KMC kernels' code in geometry `geom` (0 row form, 1 pair form, 2/3/4 wave form
+    G = 1/2/4)."""
     Q = np.asarray(Q, np.float32).reshape(-1, 3)
     N = np.asarray(N)
     n = Q.shape[0]
@@ -335,17 +336,18 @@ def probe_com(Q, N, geom, device=0):
 def probe_scan(values, form=1, device=0):
     """Canonical scan tree on the device.  form 1: one chain (len <= 1024) -> (inclusive
     sums, total).  form 0: values shaped (4, n<=64), four row-form chains -> (sums (4, n),
-    totals (4,))."""
+    totals (4,)); form 2: values shaped (2, n<=128), two pair-form chains."""
     v = np.ascontiguousarray(values, np.float32)
     if form == 1:
         n = v.size
         out = np.zeros(n + 1, np.float32)
         _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, 1))
         return out[:n], out[n]
-    v = v.reshape(4, -1)
+    rows = 2 if form == 2 else 4
+    v = v.reshape(rows, -1)
     n = v.shape[1]
-    out = np.zeros((4, n + 1), np.float32)
-    _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, 0))
+    out = np.zeros((rows, n + 1), np.float32)
+    _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, int(form)))
     return out[:, :n], out[:, n]
 
 

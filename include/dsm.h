@@ -153,13 +153,15 @@ int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* out24);
 /* Building-block probes (run one kernel on the device; no counterpart in the
    reference): the canonical FP32 scan tree of the KMC kernels (form 1: one chain of
    n <= 1024 positions, out = n inclusive sums + total; form 0: four chains of
-   n <= 64 positions in the row form, in = 4*n values, out = 4*(n+1)), exp/log/sin/cos
-   (which 0..3), Philox (6 words in, 4 out). */
+   n <= 64 positions in the row form, in = 4*n values, out = 4*(n+1); form 2: two
+   chains of n <= 128 positions in the pair form, in = 2*n, out = 2*(n+1)),
+   exp/log/sin/cos (which 0..3), Philox (6 words in, 4 out). */
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form);
 int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which);
 int dsm_probe_philox(int32_t device_id, const uint32_t* ctr_key, uint32_t* out, int32_t n);
 /* MSD centre-of-mass sample of one chain state (relative to R1) in KMC geometry `geom`
-   (0 row form, n <= 64; 1/2/3 wave form, n <= 256/512/1024): in[4i..4i+3] = Qx Qy Qz N
+   (0 row form, n <= 64; 1 pair form, n <= 128; 2/3/4 wave form, n <= 256/512/1024):
+   in[4i..4i+3] = Qx Qy Qz N
    of strand i, in[4n] = Nk; out = 3 floats. */
 int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom);
 /* Plain device-to-device copy of n_bytes (multiple of 16), timed over `reps` launches

@@ -68,6 +68,12 @@ def test_scan_tree_matches_oracle(oracle):
         for r in range(4):
             assert np.array_equal(_bits(gpu[r]), _bits(oracle.scan_tree(vals[r])))
             assert _bits(np.float32(tot[r])) == _bits(np.float32(oracle.tree_total(vals[r])))
+    for n in (1, 7, 63, 64, 65, 100, 127, 128):
+        vals = (rng.exponential(size=(2, n)) * 50).astype(np.float32)
+        gpu, tot = g.probe_scan(vals, form=2)
+        for r in range(2):
+            assert np.array_equal(_bits(gpu[r]), _bits(oracle.scan_tree(vals[r])))
+            assert _bits(np.float32(tot[r])) == _bits(np.float32(oracle.tree_total(vals[r])))
 
 
 @pytest.mark.parametrize("which", ["exp", "log", "sin2pi", "cos2pi"])
@@ -110,6 +116,7 @@ KMC_CASES = [
     dict(n=200, nk=100, beta=1.0, cd=1, events=300),
     dict(n=127, nk=50, beta=0.5, cd=1, events=300),
     dict(n=96, nk=160, beta=1.0, cd=1, events=250),
+    dict(n=64, nk=240, beta=1.0, cd=1, events=300),  # Z ~ 120: pair form (Z < 128) and wave form, crossings
     dict(n=48, nk=300, beta=1.0, cd=1, events=200),
     dict(n=9, nk=1000, beta=0.5, cd=1, events=60),
     dict(n=5, nk=2000, beta=0.5, cd=1, events=30),   # Z ~ 1330: the Z <= 2048 wave geometry
@@ -216,8 +223,9 @@ def test_gt_samples_and_correlator(engine, oracle):
     engine.remove_ensemble(ens)
 
 
-@pytest.mark.parametrize("Z,geom", [(1, 0), (2, 0), (60, 0), (64, 0), (65, 1), (250, 1), (256, 1), (257, 2),
-                                     (268, 2), (500, 2), (512, 2), (513, 3), (900, 3), (1024, 3)])
+@pytest.mark.parametrize("Z,geom", [(1, 0), (2, 0), (60, 0), (64, 0), (1, 1), (63, 1), (65, 1), (100, 1),
+                                     (128, 1), (65, 2), (250, 2), (256, 2), (257, 3), (268, 3), (500, 3),
+                                     (512, 3), (513, 4), (900, 4), (1024, 4)])
 def test_com_sample_matches_oracle(oracle, Z, geom):
     """The centre-of-mass sample of the kernels, in every geometry, against the oracle."""
     import gpu_dsm_amd as g
