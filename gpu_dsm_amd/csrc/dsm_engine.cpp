@@ -322,8 +322,8 @@ struct Ens {
 struct dsm_engine {
   int device = 0;
   hipStream_t st = nullptr;
-  hipStream_t xs[kNumBins] = {};  // xs[b], b >= 1: geometry b of a concurrent pass (default priority)
-  hipEvent_t ev_split = nullptr, ev_bin[kNumBins] = {};
+  hipStream_t xs[4] = {};  // xs[1..3]: the other streams of a concurrent pass (xs[0] unused: st)
+  hipEvent_t ev_split = nullptr, ev_bin[4] = {};
   hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};  // concurrent pass: start, row start/end, end
   hipEvent_t lev[2 * kNumBins] = {};                        // sequential pass: each launch's start/end
   int n_cu = 0;
@@ -637,18 +637,23 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   HIPCHK(launch_split_geoms(list(0), ps.n0, e->ch.Z, e->ch.stall, list(kSplitList), (int)cap, cnt + kSplitList,
                             e->st));
   HIPCHK(hipEventRecord(e->ev_split, e->st));
-  // round 1
+  // round 1.  A process has four hardware queues on this device, so four streams: the
+  // row form and the pair form on their own; the two long-chain geometries (few chains,
+  // latency-bound runs) start at once on the other two, each followed by a wave-form
+  // bulk geometry (G = 4 then G = 2, G = 8 then G = 1).  Measured on the polydisperse
+  // melt, this keeps all four queues busy to about the same end.
+  static const int kStreamOf[kNumBins] = {0, 3, 2, 1, 1, 2};
+  for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
   for (int b = ps.bin_max; b >= 1; --b) {
-    HIPCHK(hipStreamWaitEvent(e->xs[b], e->ev_split, 0));
     bind(kSplitList + b, late_of(b));
-    HIPCHK(launch_kmc(a, b, false, grid(b), e->xs[b]));
-    HIPCHK(hipEventRecord(e->ev_bin[b], e->xs[b]));
+    HIPCHK(launch_kmc(a, b, false, grid(b), e->xs[kStreamOf[b]]));
   }
+  for (int k = 1; k < 4; ++k) HIPCHK(hipEventRecord(e->ev_bin[k], e->xs[k]));
   bind(kSplitList, late_of(0));
   HIPCHK(hipEventRecord(e->cev[1], e->st));
   HIPCHK(launch_kmc(a, 0, false, grid(0), e->st));
   HIPCHK(hipEventRecord(e->cev[2], e->st));
-  for (int b = 1; b <= ps.bin_max; ++b) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[b], 0));
+  for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[k], 0));
   // round 2: chains that outgrew their geometry, through the larger ones in order
   for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) {
     bind(kLateList + b, late_of(b));
@@ -912,8 +917,8 @@ int dsm_create(int32_t device_id, dsm_engine** out) {
   e->device = device_id;
   e->n_cu = prop.multiProcessorCount;
   bool streams_ok = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) == hipSuccess;
-  for (int b = 1; b < kNumBins; ++b)
-    streams_ok = streams_ok && hipStreamCreateWithFlags(&e->xs[b], hipStreamNonBlocking) == hipSuccess;
+  for (int k = 1; k < 4; ++k)
+    streams_ok = streams_ok && hipStreamCreateWithFlags(&e->xs[k], hipStreamNonBlocking) == hipSuccess;
   if (!streams_ok) {
     dsm_destroy(e);
     return fail(DSM_ERR_DEVICE, "stream creation failed");

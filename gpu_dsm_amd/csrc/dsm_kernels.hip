@@ -113,9 +113,14 @@ __global__ void set_ev_stop_kernel(const uint64_t* ev, uint64_t* ev_stop, int ch
   if (i < n) ev_stop[chain0 + i] = ev[chain0 + i] + add;
 }
 
-// Chains of a pass into per-geometry lists by their current Z: geometry b holds a chain
-// that can still jump in it (Z below its capacity); stalled chains go to geometry 0,
-// which skips them.  Lists are cap entries apart, counts[b] their lengths.
+// Chains of a pass into per-geometry lists by their current Z.  Geometry b takes a chain
+// whose Z is below its capacity -- for the wave-form geometries (b >= 2) below 7/8 of it,
+// so that a long chain, whose Z drifts by a few tens over a window, rarely outgrows its
+// geometry mid-pass (it would then finish alone in the late round, after everything
+// else; measured on the polydisperse melt, the same margin at the row and pair
+// boundaries costs more in the first round than it saves in the late one).  Stalled
+// chains go to geometry 0, which skips them.  Lists are cap entries apart, counts[b]
+// their lengths.
 __global__ void split_geoms_kernel(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
                                    int* counts) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,7 +130,7 @@ __global__ void split_geoms_kernel(const int* src, int n, const int* Z, const in
   const int z = Z[c];
   int b = 0;
   if (stall[c] == 0)
-    while (b + 1 < kNumGeoms && z >= kmc_geom_capacity(b)) ++b;
+    while (b + 1 < kNumGeoms && z >= kmc_geom_capacity(b) - (b >= 2 ? kmc_geom_capacity(b) / 8 : 0)) ++b;
   lists[(int64_t)b * cap + atomicAdd(counts + b, 1)] = c;
 }
 
