@@ -1085,13 +1085,13 @@ __device__ __forceinline__ void store_strands(const Strands<G>& C, const ChainAr
 // each reaches the ensemble's t_end (or its jump target ev_stop, or one jump in the
 // plain form), pushing chains that outgrow the geometry to the overflow list.
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
-__device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, lds_f* stg, lds_f* ltab, int& lds_ens) {
+__device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entries, lds_f* stg, lds_f* ltab,
+                                         int& lds_ens) {
   constexpr int CAP = Geo<CPW, G>::CAP;
   constexpr int SEG = Geo<CPW, G>::SEG;
   const LaneC<CPW> L;
   const int c0 = __builtin_amdgcn_readfirstlane(a.work[CPW * unit]);
   const int ens = __builtin_amdgcn_readfirstlane(a.ch.ens[c0]);
-  const int n_entries = *a.n_work;  // lists of the larger geometries are not padded
   EnsLocal E;
   load_ens(E, a.ens, ens);
   Tabs<LT> T;
@@ -1329,13 +1329,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((EXT ||
     run_stream<CPW, G, LT, EXT>(a, stg, ltab);
   } else {
     int lds_ens = -1;
-    const int n_units = (*a.n_work + CPW - 1) / CPW;
+    // the list length is read once: its counter shares a cache line with the queue
+    // counter every wave increments (lists of the larger geometries are not padded)
+    const int n_entries = *a.n_work;
+    const int n_units = (n_entries + CPW - 1) / CPW;
     for (;;) {
       int unit = 0;
       if (lane_id() == 0) unit = atomicAdd(a.queue, 1);
       unit = __builtin_amdgcn_readfirstlane(unit);
       if (unit >= n_units) break;
-      run_unit<CPW, G, LT, PLAIN, EXT>(a, unit, stg, ltab, lds_ens);
+      run_unit<CPW, G, LT, PLAIN, EXT>(a, unit, n_entries, stg, ltab, lds_ens);
     }
   }
 }
