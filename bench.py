@@ -147,8 +147,9 @@ def main():
         "config": {"workload": "monodisperse equilibrium melt, %d chains, Nk=%d, beta=%g, CD on, G(t) multi-tau "
                                "p=%d m=%d, %d tau_K per step" % (args.chains, w["nk"], w["beta"], w["corr_p"],
                                                                  w["corr_m"], args.window),
-                   "form": "batched persistent (row form: four register-resident chains per wave; "
-                           "wave form for Z > 63)",
+                   "form": "batched persistent (row form: four register-resident chains per wave, "
+                           "16-lane segments refilled one chain at a time; wave form for the chains "
+                           "that reach Z = 64)",
                    "n_chains": args.chains, "nk": w["nk"], "beta": w["beta"], "cd": w["cd"],
                    "window_tau_k": args.window},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
