@@ -878,15 +878,25 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   }
   const bool rare = kind >= kEvCDd && kind <= kEvSDdTail;
   if (__ballot(rare)) {
-    // values the rare effects read (cross-lane: run on the whole wave)
+    // values the rare effects read (cross-lane: run on the whole wave).  The EXT
+    // variants read each group only when a chain of the wave needs it (the strand
+    // vectors at p, p+1 for CD events, the random numbers for creations): that keeps
+    // them within three waves per SIMD; the lean ones read them unconditionally, which
+    // measured slightly faster.
     const int pq = p < CAP - 1 ? p : CAP - 2;
-    const float qxp = value_at<CPW, G>(C.qx, pq, L), qyp = value_at<CPW, G>(C.qy, pq, L);
-    const float qzp = value_at<CPW, G>(C.qz, pq, L);
-    const float qxp1 = value_at<CPW, G>(C.qx, pq + 1, L), qyp1 = value_at<CPW, G>(C.qy, pq + 1, L);
-    const float qzp1 = value_at<CPW, G>(C.qz, pq + 1, L);
-    const float u_tau = seg_get<CPW>(RC.tau, k_rng - 1, L);
-    const float g0 = seg_get<CPW>(RC.g0, k_rng - 1, L), g1 = seg_get<CPW>(RC.g1, k_rng - 1, L);
-    const float g2 = seg_get<CPW>(RC.g2, k_rng - 1, L);
+    float qxp = 0.0f, qyp = 0.0f, qzp = 0.0f, qxp1 = 0.0f, qyp1 = 0.0f, qzp1 = 0.0f;
+    if (!EXT || __ballot(kind == kEvCDd || kind == kEvCDc)) {
+      qxp = value_at<CPW, G>(C.qx, pq, L); qyp = value_at<CPW, G>(C.qy, pq, L);
+      qzp = value_at<CPW, G>(C.qz, pq, L);
+      qxp1 = value_at<CPW, G>(C.qx, pq + 1, L); qyp1 = value_at<CPW, G>(C.qy, pq + 1, L);
+      qzp1 = value_at<CPW, G>(C.qz, pq + 1, L);
+    }
+    float u_tau = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    if (!EXT || __ballot(kind == kEvCDc || kind == kEvCDcTail || kind == kEvSDcHead || kind == kEvSDcTail)) {
+      u_tau = seg_get<CPW>(RC.tau, k_rng - 1, L);
+      g0 = seg_get<CPW>(RC.g0, k_rng - 1, L); g1 = seg_get<CPW>(RC.g1, k_rng - 1, L);
+      g2 = seg_get<CPW>(RC.g2, k_rng - 1, L);
+    }
     const int N0n = S.geti(kStN, 0), N1n = S.geti(kStN, 1);
     // polydisperse background: the new link's partner quantile is the top bits of word 3
     // of this jump's Philox draw (H.ev already counts the jump)
@@ -1321,7 +1331,7 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
 // centre-of-mass samples) and the polydisperse CD background (partner tables).
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((EXT || PLAIN || CPW == 1) ? 1 : 3))) kmc_kernel(KmcArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((PLAIN || CPW == 1) ? 1 : 3))) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<CPW, G>();

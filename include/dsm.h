@@ -71,8 +71,9 @@ typedef struct {
 typedef struct {
   uint64_t events;      /* KMC jumps performed */
   uint64_t z_sum;       /* sum over jumps of the chain's Z (algorithmic byte model) */
-  double kmc_ms;        /* device time of the KMC kernel launches (HIP events) */
-  double first_kmc_ms;  /* device time of the first-bin (slot count 1 or the smallest) launches only */
+  double kmc_ms;        /* device time of the KMC passes (HIP events: the sum of the launches of a
+                           sequential pass; the span from split to late round of a concurrent one) */
+  double first_kmc_ms;  /* device time of the row-form (geometry 0) launches only */
   double total_ms;      /* host wall time of the call */
   int64_t kmc_launches;
   int64_t first_launches;
