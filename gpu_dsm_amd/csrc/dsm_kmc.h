@@ -1331,7 +1331,7 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
 // centre-of-mass samples) and the polydisperse CD background (partner tables).
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((PLAIN || CPW == 1) ? 1 : 3))) kmc_kernel(KmcArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((PLAIN || (CPW == 1 && G != 2)) ? 1 : (CPW == 1 ? 2 : 3)))) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<CPW, G>();
