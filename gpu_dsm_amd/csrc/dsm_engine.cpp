@@ -579,7 +579,7 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   for (int id : ids) {
     z_est += e->ens[id].z_sum_est;
     n_all += e->ens[id].n;
-    ext |= (e->ens[id].P.sample_mode == kSampleMsd || e->ens[id].P.partner) ? 1 : 0;
+    ext |= (e->ens[id].P.sample_mode == kSampleMsd || e->ens[id].P.partner || e->ens[id].P.flow) ? 1 : 0;
     n0 += (e->ens[id].n + 3) & ~3;
     bin_max = std::max(bin_max, e->ens[id].bin_max);
     tab_max = std::max(tab_max, 9 * (e->ens[id].nmax + 2) + 4 * e->ens[id].nmax + 4);

@@ -1000,13 +1000,13 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
       const int ao = gap > 0 ? at : CAP;
       gap_open<CPW, G>(C.qx, ao, Z, L); gap_open<CPW, G>(C.qy, ao, Z, L); gap_open<CPW, G>(C.qz, ao, Z, L);
       gap_open<CPW, G>(C.q2, ao, Z, L); gap_open<CPW, G>(C.w, ao, Z, L); gap_open<CPW, G>(C.n, ao, Z, L);
-      gap_lds<CPW, G>(S, 1, ao, Z, L);
+      if (!EXT || !E.flow) gap_lds<CPW, G>(S, 1, ao, Z, L);
     }
     if (any_close) {
       const int ac = gap < 0 ? at : CAP;
       gap_close<CPW, G>(C.qx, ac, Z, L); gap_close<CPW, G>(C.qy, ac, Z, L); gap_close<CPW, G>(C.qz, ac, Z, L);
       gap_close<CPW, G>(C.q2, ac, Z, L); gap_close<CPW, G>(C.w, ac, Z, L); gap_close<CPW, G>(C.n, ac, Z, L);
-      gap_lds<CPW, G>(S, -1, ac, Z, L);
+      if (!EXT || !E.flow) gap_lds<CPW, G>(S, -1, ac, Z, L);
     }
     // patches
     const float q0 = q2_of(p0x, p0y, p0z), q1 = q2_of(p1x, p1y, p1z);
@@ -1027,11 +1027,17 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
     H.Z = znew;
     stage_qw<CPW, G>(S, C, L);
   }
-  // 5. refresh the pairs next to the change
+  // 5. refresh the pairs next to the change.  Under flow every pair rate is recomputed
+  // from the deformed strands before the next jump (the affine step follows every jump),
+  // so the EXT variant, which the passes with a flow ensemble launch, neither shifts the
+  // cached rates (gap_lds above) nor patches them here; the lean variant keeps the
+  // unconditional code, measured 1.3% faster for equilibrium passes.
   stage_n<CPW, G>(S, C, L);
   wave_lds_sync();
-  refresh_pairs<CPW, G, LT>(S, T, E.cd, H.Z, lo, hi, zero_at, L);
-  wave_lds_sync();
+  if (!EXT || !E.flow) {
+    refresh_pairs<CPW, G, LT>(S, T, E.cd, H.Z, lo, hi, zero_at, L);
+    wave_lds_sync();
+  }
 }
 
 // ------------------------------------------------------------------ load / store
@@ -1329,7 +1335,8 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 // PLAIN: one jump per chain (the per-launch form); otherwise run to the targets.
 // EXT: the variant with the less common features, launched when an ensemble of the pass
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
-// centre-of-mass samples) and the polydisperse CD background (partner tables).
+// centre-of-mass samples), the polydisperse CD background (partner tables) and flow
+// (no incremental rate refresh: the affine step recomputes every rate).
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((PLAIN || (CPW == 1 && G != 2)) ? 1 : (CPW == 1 ? 2 : 3)))) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];

@@ -11,8 +11,8 @@ namespace dsm {
 // 1 = pair form (two chains per wave, Z <= 128), 2/3/4/5 = wave form (one chain per
 // wave, Z <= 256/512/1024/2048): geometry g holds 64 << g positions per chain.  Persistent: grid =
 // blocks of one wave; plain = one jump per chain.  lds_floats > 0 stages the rate
-// tables (that many floats) in LDS.  a.ext selects the variant with MSD(t) tracking and
-// polydisperse CD partners.
+// tables (that many floats) in LDS.  a.ext selects the variant with MSD(t) tracking,
+// polydisperse CD partners and the flow-mode shortcut.
 constexpr int kNumGeoms = 6;
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st);
 int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext);

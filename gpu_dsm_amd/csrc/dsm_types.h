@@ -84,7 +84,7 @@ struct KmcArgs {
   float* stage;         // sample staging arena
   int use_ev_stop;      // stop each chain when ev reaches ch.ev_stop[c]
   int lds_floats;       // per-wave LDS table copy size (0: tables read from global memory)
-  int ext;              // an ensemble of the pass samples MSD(t) or has a polydisperse CD background
+  int ext;              // an ensemble of the pass samples MSD(t), has a polydisperse CD background or flows
 };
 
 }  // namespace dsm
