@@ -121,17 +121,30 @@ __global__ void set_ev_stop_kernel(const uint64_t* ev, uint64_t* ev_stop, int ch
 // boundaries costs more in the first round than it saves in the late one).  Stalled
 // chains go to geometry 0, which skips them.  Lists are cap entries apart, counts[b]
 // their lengths.
+// One atomic per wave and geometry: the lanes of a geometry get consecutive slots.
 __global__ void split_geoms_kernel(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
                                    int* counts) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = src[i];
-  if (c < 0) return;
-  const int z = Z[c];
-  int b = 0;
-  if (stall[c] == 0)
-    while (b + 1 < kNumGeoms && z >= kmc_geom_capacity(b) - (b >= 2 ? kmc_geom_capacity(b) / 8 : 0)) ++b;
-  lists[(int64_t)b * cap + atomicAdd(counts + b, 1)] = c;
+  const int c = i < n ? src[i] : -1;
+  int b = -1;
+  if (c >= 0) {
+    const int z = Z[c];
+    b = 0;
+    if (stall[c] == 0)
+      while (b + 1 < kNumGeoms && z >= kmc_geom_capacity(b) - (b >= 2 ? kmc_geom_capacity(b) / 8 : 0)) ++b;
+  }
+  const int lane = (int)__lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int g = 0; g < kNumGeoms; ++g) {
+    const uint64_t m = __ballot(b == g);
+    if (m == 0) continue;
+    const int leader = (int)__builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counts + g, __popcll(m));
+    base = __shfl(base, leader);
+    if (b == g) lists[(int64_t)g * cap + base + __popcll(m & below)] = c;
+  }
 }
 
 __global__ void iota_kernel(int* p, int start, int n) {
