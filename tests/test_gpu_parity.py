@@ -361,3 +361,38 @@ def test_many_small_ensembles_batch(engine, oracle):
         assert_same_state(engine.state(ens), O.state(), "ensemble %d" % ens)
     for ens in ids:
         engine.remove_ensemble(ens)
+
+
+@pytest.mark.parametrize("sample", ["gt", "msd"])
+def test_correlator_beside_wave_form_tail(engine, oracle, sample):
+    """Windows in which chains outgrow the row form (Nk = 100): the G(t) / MSD(t) update of
+    the chains the row form finished runs beside the wave-form tail, the handed-on
+    chains' update after it.  Every chain must be updated exactly once per window:
+    samples bit-exact and the correlator equal to the oracle's over the same samples."""
+    n, nk, T, win = 2048, 100, 48.0, 16
+    p, m = 16, 2
+    s = oracle.corr_levels(T, 1.0, p, m)
+    P = oracle.Params(1.0, nk, 1)
+    ens = engine.add_ensemble(n, nk, 1.0, 1, seed=17, sample=sample, delta=1.0, corr_p=p, corr_m=m,
+                              corr_levels=s, window=win)
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=17, n=n)
+    _, samples = O.advance(t_end=T, delta=1.0, k0=0, k_end=int(T), samples=True, msd=(sample == "msd"))
+    z_max = 0
+    t = 0.0
+    while t < T:
+        t = min(T, t + win)
+        engine.advance(ens, t)
+        z_max = max(z_max, int(engine.state(ens)["Z"].max()))
+    assert z_max >= 64, "no chain reached the wave form (max Z %d)" % z_max
+    assert_same_state(engine.state(ens), O.state(), "%s run" % sample)
+    k0, last = engine.last_samples(ens)
+    assert np.array_equal(_bits(last), _bits(samples[k0:k0 + last.shape[0]]))
+    corr = oracle.Correlator(p, m, s, n, type=1 if sample == "msd" else 0)
+    for row in samples:
+        corr.add_row(row)
+    lag_o, g_o = corr.result()
+    lag, g = engine.correlator(ens)
+    assert np.array_equal(lag, lag_o)
+    np.testing.assert_allclose(g, g_o, rtol=1e-10, atol=1e-12)
+    engine.remove_ensemble(ens)
