@@ -258,6 +258,80 @@ __device__ __forceinline__ void corr_insert_from(const CorrArgs& a, int c, int l
   }
 }
 
+// As corr_insert_from for p = P, with the lag loop in groups of four lags whose shift
+// register values and lag sums are loaded together: the loads of a group are in flight
+// at once instead of one lag after another (the lag sums and the shift register are
+// distinct arrays, which the compiler cannot assume).  Same operations in the same order.
+template <int P>
+__device__ __forceinline__ void corr_insert_from_p(const CorrArgs& a, int c, int lvl, double vx, double vy, double vz,
+                                                   int* ins, int* macc, int* nins) {
+  const int64_t nch = a.nch;
+  const int m = a.m;
+  for (;;) {
+    if (lvl == a.s) break;
+    const int in = ins[lvl];
+    const int nin = nins[lvl];
+    double* __restrict__ D = a.D + (int64_t)(lvl * P) * 3 * nch + c;
+    double* __restrict__ A = a.A + (int64_t)(lvl * 3) * nch + c;
+    double* __restrict__ Cc = a.C + (int64_t)(lvl * P) * nch + c;
+    const int jmin = (lvl == 0) ? 0 : P / m;
+    // lag j pairs v with the value inserted j steps before it (j = 0: v itself)
+#pragma unroll
+    for (int j0 = 0; j0 < P; j0 += 4) {
+      if (j0 + 3 < jmin || j0 > nin) continue;
+      double dx[4], dy[4], dz[4], cv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int j = j0 + k;
+        int idx = in - j;
+        if (idx < 0) idx += P;
+        const bool use = j >= jmin && j <= nin;
+        dx[k] = vx; dy[k] = vy; dz[k] = vz; cv[k] = 0.0;
+        if (use && j > 0) {
+          dx[k] = D[(int64_t)(idx * 3 + 0) * nch];
+          dy[k] = D[(int64_t)(idx * 3 + 1) * nch];
+          dz[k] = D[(int64_t)(idx * 3 + 2) * nch];
+        }
+        if (use) cv[k] = Cc[(int64_t)j * nch];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int j = j0 + k;
+        if (!(j >= jmin && j <= nin)) continue;
+        if (a.type == 0) {
+          cv[k] += vx * dx[k] + vy * dy[k] + vz * dz[k];
+        } else {
+          const double ex = vx - dx[k], ey = vy - dy[k], ez = vz - dz[k];
+          cv[k] += ex * ex + ey * ey + ez * ez;
+        }
+        Cc[(int64_t)j * nch] = cv[k];
+      }
+    }
+    D[(int64_t)(in * 3 + 0) * nch] = vx;
+    D[(int64_t)(in * 3 + 1) * nch] = vy;
+    D[(int64_t)(in * 3 + 2) * nch] = vz;
+    // MSD carries the first value of each block up unaveraged
+    // (common/eq_ensemble_kernel.cu:181-189)
+    double ax = A[0], ay = A[nch], az = A[2 * nch];
+    if (a.type == 0 || macc[lvl] == 0) { ax += vx; ay += vy; az += vz; }
+    macc[lvl]++;
+    const bool push = macc[lvl] == m;
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    if (push) {
+      if (a.type == 0) { ux = ax / m; uy = ay / m; uz = az / m; }
+      else { ux = ax; uy = ay; uz = az; }
+      ax = 0.0; ay = 0.0; az = 0.0;
+      macc[lvl] = 0;
+    }
+    A[0] = ax; A[nch] = ay; A[2 * nch] = az;
+    ins[lvl] = (in + 1 == P) ? 0 : in + 1;
+    nins[lvl] = nin < P ? nin + 1 : P;
+    if (!push) break;
+    ++lvl;
+    vx = ux; vy = uy; vz = uz;
+  }
+}
+
 // Any p: every level in device memory.
 __global__ void __launch_bounds__(256) corr_update_kernel(CorrArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -277,7 +351,7 @@ __global__ void __launch_bounds__(256) corr_update_kernel(CorrArgs a) {
 // the block pushes in device memory as before.  Every sum is formed in the same order
 // as in corr_update_kernel, so both give identical results.
 template <int P>
-__global__ void __launch_bounds__(256) corr_update_reg_kernel(CorrArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) corr_update_reg_kernel(CorrArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.nch) return;
   int ins[40], macc[40], nins[40];
@@ -331,7 +405,7 @@ __global__ void __launch_bounds__(256) corr_update_reg_kernel(CorrArgs a) {
     hx[0] = vx; hy[0] = vy; hz[0] = vz;
     in0 = (in0 + 1 == P) ? 0 : in0 + 1;
     nins0 = nins0 < P ? nins0 + 1 : P;
-    if (push) corr_insert_from(a, c, 1, ux, uy, uz, ins, macc, nins);
+    if (push) corr_insert_from_p<P>(a, c, 1, ux, uy, uz, ins, macc, nins);
   }
   // write back: the slots of this window's values (the newest min(nrows, P)), the lag
   // sums and the block accumulator
