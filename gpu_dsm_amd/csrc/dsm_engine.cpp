@@ -570,6 +570,7 @@ struct Pass {
   int n0 = 0;  // entries of the bin-0 list
   int bin_max = 0;
   bool concurrent = false;  // batched passes only: split by Z, geometries side by side
+  int n_ens = 0;
   KmcArgs a{};
 };
 
@@ -586,6 +587,7 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   }
   ps.n0 = n0;
   ps.bin_max = bin_max;
+  ps.n_ens = (int)ids.size();
   // geometries side by side when chains beyond the first wave-form geometry can occur
   // (their runs are long and few) or when many chains are expected above the row form's
   // 63 positions (the pair form then carries the bulk)
@@ -685,11 +687,20 @@ int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs
   seq[ns++] = 0;
   if (ps.bin_max >= 1)
     for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) seq[ns++] = b;
+  // batched, one ensemble: the row form takes its chains in decreasing Z, so the chains
+  // started last, which set the end of the launch, are the shortest (several ensembles
+  // keep their order: a wave runs one ensemble at a time)
+  const bool zorder = !plain && ps.n_ens == 1;
+  constexpr int kZList = kNumLists - 1;
+  if (zorder)
+    HIPCHK(launch_z_order(e->d_work, ps.n0, e->ch.Z, e->ch.stall, cnt + 2 * kNumLists,
+                          e->d_work + (size_t)kZList * e->d_work_cap, cnt + kZList, e->st));
   for (int k = 0; k < ns; ++k) {
     const int b = seq[k], nb = k + 1 < ns ? seq[k + 1] : b + 1;
-    a.work = e->d_work + (size_t)b * e->d_work_cap;
-    a.n_work = cnt + b;
-    a.queue = cnt + kNumLists + b;
+    const int in = (b == 0 && zorder) ? kZList : b;
+    a.work = e->d_work + (size_t)in * e->d_work_cap;
+    a.n_work = cnt + in;
+    a.queue = cnt + kNumLists + in;
     a.overflow = e->d_work + (size_t)nb * e->d_work_cap;
     a.n_overflow = cnt + nb;
     const int grid = device_blocks(e, b, plain, a.lds_floats, a.ext != 0);
@@ -929,7 +940,7 @@ int dsm_create(int32_t device_id, dsm_engine** out) {
   HIPCHK(hipEventCreateWithFlags(&e->ev_split, hipEventDisableTiming));
   for (auto& ev : e->ev_bin) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   int rc;
-  if ((rc = dev_alloc(&e->d_cnt, 2 * kNumLists)) || (rc = dev_alloc(&e->d_counters, 4)) ||
+  if ((rc = dev_alloc(&e->d_cnt, 2 * kNumLists + 32)) || (rc = dev_alloc(&e->d_counters, 4)) ||
       (rc = dev_alloc(&e->d_healthy, 1))) {
     dsm_destroy(e);
     return rc;

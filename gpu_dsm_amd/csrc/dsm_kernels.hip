@@ -147,6 +147,63 @@ __global__ void split_geoms_kernel(const int* src, int n, const int* Z, const in
   }
 }
 
+// Row-form work list in decreasing Z (16 buckets of four; Z >= 64 first, stalled chains
+// last): the chains started last, at the end of a pass, are the shortest ones.
+__device__ __forceinline__ int zbucket(const int* Z, const int* stall, int c) {
+  if (stall[c] != 0) return 16;
+  const int z = Z[c];
+  return z >= 64 ? 0 : 15 - (z >> 2);
+}
+constexpr int kZBuckets = 17;
+
+// Block-local histograms in LDS, one global atomic per block and bucket (a per-element
+// global atomic on 17 counters serialises in L2).  Grid-stride over the elements.
+__global__ void __launch_bounds__(1024) zhist_kernel(const int* src, int n, const int* Z, const int* stall, int* hist) {
+  __shared__ int lh[kZBuckets];
+  if (threadIdx.x < kZBuckets) lh[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = src[i];
+    if (c >= 0) atomicAdd(lh + zbucket(Z, stall, c), 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kZBuckets && lh[threadIdx.x]) atomicAdd(hist + threadIdx.x, lh[threadIdx.x]);
+}
+
+// hist -> exclusive offsets (in place), total into *n_out
+__global__ void zscan_kernel(int* hist, int* n_out) {
+  if (threadIdx.x != 0) return;
+  int acc = 0;
+  for (int g = 0; g < kZBuckets; ++g) { const int h = hist[g]; hist[g] = acc; acc += h; }
+  *n_out = acc;
+}
+
+// Each block reserves its range of every bucket with one atomic, then places its
+// elements with LDS atomics (order within a bucket is arbitrary).
+__global__ void __launch_bounds__(1024) zscatter_kernel(const int* src, int n, const int* Z, const int* stall,
+                                                        int* cursor, int* dst) {
+  __shared__ int lh[kZBuckets], base[kZBuckets];
+  if (threadIdx.x < kZBuckets) lh[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = src[i];
+    if (c >= 0) atomicAdd(lh + zbucket(Z, stall, c), 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kZBuckets) {
+    base[threadIdx.x] = lh[threadIdx.x] ? atomicAdd(cursor + threadIdx.x, lh[threadIdx.x]) : 0;
+    lh[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = src[i];
+    if (c >= 0) {
+      const int b = zbucket(Z, stall, c);
+      dst[base[b] + atomicAdd(lh + b, 1)] = c;
+    }
+  }
+}
+
 __global__ void iota_kernel(int* p, int start, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = start + i;
@@ -624,6 +681,17 @@ hipError_t launch_split_geoms(const int* src, int n, const int* Z, const int* st
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(split_geoms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, src, n, Z, stall, lists, cap,
                      counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_z_order(const int* src, int n, const int* Z, const int* stall, int* scratch17, int* dst,
+                          int* n_out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, (uint32_t*)scratch17, 0u, (int64_t)kZBuckets);
+  const int blocks = n > 64 * 1024 * 4 ? 256 : (n + 4095) / 4096 + 1;  // a few elements per thread
+  hipLaunchKernelGGL(zhist_kernel, dim3(blocks), dim3(1024), 0, st, src, n, Z, stall, scratch17);
+  hipLaunchKernelGGL(zscan_kernel, dim3(1), dim3(64), 0, st, scratch17, n_out);
+  hipLaunchKernelGGL(zscatter_kernel, dim3(blocks), dim3(1024), 0, st, src, n, Z, stall, scratch17, dst);
   return hipGetLastError();
 }
 
