@@ -322,7 +322,7 @@ struct Ens {
 struct dsm_engine {
   int device = 0;
   hipStream_t st = nullptr;
-  hipStream_t xs[4] = {};  // xs[1..3]: the other streams of a concurrent pass (xs[0] unused: st)
+  hipStream_t xs[4] = {};  // xs[1..3]: the other streams of a concurrent pass (xs[0] stays null; st is stream 0)
   hipEvent_t ev_split = nullptr, ev_bin[4] = {};
   hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};  // concurrent pass: start, row start/end, end
   hipEvent_t lev[2 * kNumBins] = {};                        // sequential pass: each launch's start/end
@@ -643,12 +643,14 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // row form and the pair form on their own; the two long-chain geometries (few chains,
   // latency-bound runs) start at once on the other two, each followed by a wave-form
   // bulk geometry (G = 4 then G = 2, G = 8 then G = 1).  Measured on the polydisperse
-  // melt, this keeps all four queues busy to about the same end.
+  // melt, this ends the window soonest of the mappings tried (269 ms; with G = 1 and
+  // both long-chain geometries started at once and G = 2 before the pair form, 280 ms).
   static const int kStreamOf[kNumBins] = {0, 3, 2, 1, 1, 2};
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
+  auto stream = [&](int k) { return k == 0 ? e->st : e->xs[k]; };  // xs[0] is not a stream
   for (int b = ps.bin_max; b >= 1; --b) {
     bind(kSplitList + b, late_of(b));
-    HIPCHK(launch_kmc(a, b, false, grid(b), e->xs[kStreamOf[b]]));
+    HIPCHK(launch_kmc(a, b, false, grid(b), stream(kStreamOf[b])));
   }
   for (int k = 1; k < 4; ++k) HIPCHK(hipEventRecord(e->ev_bin[k], e->xs[k]));
   bind(kSplitList, late_of(0));
