@@ -643,8 +643,9 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // row form and the pair form on their own; the two long-chain geometries (few chains,
   // latency-bound runs) start at once on the other two, each followed by a wave-form
   // bulk geometry (G = 4 then G = 2, G = 8 then G = 1).  Measured on the polydisperse
-  // melt, this ends the window soonest of the mappings tried (269 ms; with G = 1 and
-  // both long-chain geometries started at once and G = 2 before the pair form, 280 ms).
+  // melt, this ends the window soonest of the mappings tried (269 ms; 276 and 280 ms
+  // for two others): the first round keeps the device's VALUs busy to its end, so the
+  // window is set by the total work more than by the order.
   static const int kStreamOf[kNumBins] = {0, 3, 2, 1, 1, 2};
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
   auto stream = [&](int k) { return k == 0 ? e->st : e->xs[k]; };  // xs[0] is not a stream
