@@ -73,7 +73,8 @@ typedef struct {
   uint64_t z_sum;       /* sum over jumps of the chain's Z (algorithmic byte model) */
   double kmc_ms;        /* device time of the KMC passes (HIP events: the sum of the launches of a
                            sequential pass; the span from split to late round of a concurrent one) */
-  double first_kmc_ms;  /* device time of the row-form (geometry 0) launches only */
+  double first_kmc_ms;  /* device time of each pass's first launch only (the row form; the wave form
+                           for a pass of few chains) */
   double total_ms;      /* host wall time of the call */
   int64_t kmc_launches;
   int64_t first_launches;

@@ -127,11 +127,15 @@ KMC_CASES = [
     dict(n=128, nk=40, beta=2.0, cd=1, events=300, exp_wait=True),
     dict(n=128, nk=30, beta=1.0, cd=1, events=300, kappa=[0, 0.05, 0, 0, 0, 0, 0, 0, 0]),
     dict(n=64, nk=20, beta=1.0, cd=1, events=300, kappa=[0.02, 0, 0, 0, -0.01, 0, 0, 0, -0.01]),
+    # more chains than one wave-form wave each (12 per CU): batched passes use the row form
+    dict(n=3200, nk=40, beta=2.0, cd=1, events=60, exp_wait=True),
+    dict(n=3200, nk=30, beta=1.0, cd=1, events=60, kappa=[0, 0.05, 0, 0, 0, 0, 0, 0, 0]),
+    dict(n=3200, nk=20, beta=1.0, cd=1, events=60, kappa=[0.02, 0, 0, 0, -0.01, 0, 0, 0, -0.01]),
 ]
 
 
 def _case_id(c):
-    s = "nk%d_b%g_cd%d" % (c["nk"], c["beta"], c["cd"])
+    s = "n%d_nk%d_b%g_cd%d" % (c["n"], c["nk"], c["beta"], c["cd"])
     if c.get("exp_wait"):
         s += "_expwait"
     if c.get("kappa"):
@@ -160,14 +164,14 @@ def test_kmc_parity(engine, oracle, case, mode):
 GEX = (1.5, 1.0, 40000.0, 120.0)  # a, b, mp, Mk
 
 
+@pytest.mark.parametrize("n", [600, 3200])
 @pytest.mark.parametrize("mode", ["batched", "plain"])
 @pytest.mark.parametrize("partner", ["gex", "melt"])
-def test_partner_background_parity(engine, oracle, partner, mode):
+def test_partner_background_parity(engine, oracle, partner, mode, n):
     """Polydisperse CD background: partner tables identical to the oracle's, and every
     CD lifetime drawn through them bit-exact (initial links and created ones), on a
     polydisperse ensemble whose chains use both the row and the wave form."""
     rng = np.random.default_rng(7)
-    n = 600
     nk = rng.choice([12, 30, 60, 140, 400], size=n, p=[0.3, 0.3, 0.2, 0.15, 0.05]).astype(np.int32)
     nmax = int(nk.max())
     P = oracle.Params(1.0, nmax, 1, nmax=nmax, partner=partner, gex=GEX, nk_chains=nk)
@@ -252,10 +256,12 @@ MSD_CASES = [
     dict(n=127, nk=40, beta=0.5, cd=1, T=150.0, mode="batched"),
     dict(n=64, nk=12, beta=1.0, cd=0, T=40.0, mode="plain"),
     dict(n=9, nk=400, beta=0.5, cd=1, T=15.0, mode="batched"),     # wave form (Z up to ~290)
+    dict(n=3200, nk=40, beta=0.5, cd=1, T=20.0, mode="batched"),   # row form (a pass of many chains)
 ]
 
 
-@pytest.mark.parametrize("case", MSD_CASES, ids=lambda c: "nk%d_b%g_cd%d_%s" % (c["nk"], c["beta"], c["cd"], c["mode"]))
+@pytest.mark.parametrize("case", MSD_CASES,
+                         ids=lambda c: "n%d_nk%d_b%g_cd%d_%s" % (c["n"], c["nk"], c["beta"], c["cd"], c["mode"]))
 def test_msd_samples_and_correlator(engine, oracle, case):
     """MSD(t): R1 bookkeeping, centre-of-mass samples (bit-exact) and the type-1 multi-tau
     correlator against the oracle."""
@@ -309,14 +315,15 @@ def test_flow_stress_rows(engine, oracle):
     engine.remove_ensemble(ens)
 
 
+@pytest.mark.parametrize("scale", [1, 40])
 @pytest.mark.parametrize("mode", ["batched", "plain"])
-def test_multi_ensemble_flow_batch(engine, oracle, mode):
+def test_multi_ensemble_flow_batch(engine, oracle, mode, scale):
     """Several ensembles (different shear rates, chain counts that are not multiples of
     four, one equilibrium ensemble) advanced together in one pass: each must match its
     own oracle run bit for bit, with its own flow output rows."""
     nk, T, dlt = 24, 12.0, 3.0
     rates = [0.0, 0.01, 0.1, 0.5]
-    counts = [37, 64, 5, 18]
+    counts = [37 * scale, 64 * scale, 5 * scale, 18 * scale]  # scale 40: row form in batched passes
     ids, refs = [], []
     for q, (gd, n) in enumerate(zip(rates, counts)):
         kappa = [0, gd, 0, 0, 0, 0, 0, 0, 0]
@@ -365,11 +372,11 @@ def test_many_small_ensembles_batch(engine, oracle):
 
 @pytest.mark.parametrize("sample", ["gt", "msd"])
 def test_correlator_beside_wave_form_tail(engine, oracle, sample):
-    """Windows in which chains outgrow the row form (Nk = 100): the G(t) / MSD(t) update of
-    the chains the row form finished runs beside the wave-form tail, the handed-on
-    chains' update after it.  Every chain must be updated exactly once per window:
-    samples bit-exact and the correlator equal to the oracle's over the same samples."""
-    n, nk, T, win = 2048, 100, 48.0, 16
+    """Windows in which chains outgrow the row form (Nk = 100, enough chains for the row
+    form): the chains that reach Z = 64 record the rest of their window's samples in the
+    wave form after the row form has finished.  Samples bit-exact and the correlator equal
+    to the oracle's over the same samples."""
+    n, nk, T, win = 4096, 100, 48.0, 16
     p, m = 16, 2
     s = oracle.corr_levels(T, 1.0, p, m)
     P = oracle.Params(1.0, nk, 1)
