@@ -572,6 +572,8 @@ struct Pass {
   bool concurrent = false;  // batched passes only: split by Z, geometries side by side
   int n_ens = 0;
   int n_live = 0;           // chains of the pass
+  int pending = 0;          // launch times recorded, not yet read: 1 sequential (ns launches), 2 concurrent
+  int ns = 0;
   KmcArgs a{};
 };
 
@@ -667,12 +669,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
     HIPCHK(launch_kmc(a, b, false, grid(b), e->st));
   }
   HIPCHK(hipEventRecord(e->cev[3], e->st));
-  HIPCHK(hipEventSynchronize(e->cev[3]));
-  float ms = 0.0f;
-  HIPCHK(hipEventElapsedTime(&ms, e->cev[0], e->cev[3]));
-  rs->kmc_ms += ms;
-  HIPCHK(hipEventElapsedTime(&ms, e->cev[1], e->cev[2]));
-  rs->first_kmc_ms += ms;
+  ps.pending = 2;  // read by pass_times after the caller's next synchronisation
   rs->kmc_launches += 1 + ps.bin_max + (late_last(ps.bin_max) - kLateFirst + 1);
   rs->first_launches++;
   return DSM_OK;
@@ -725,22 +722,41 @@ int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs
     if (k == 0) rs->first_launches++;
   }
   if (timed) {
-    HIPCHK(hipEventSynchronize(e->lev[2 * ns - 1]));
-    for (int k = 0; k < ns; ++k) {
-      float ms = 0.0f;
+    ps.pending = 1;  // read by pass_times after the caller's next synchronisation
+    ps.ns = ns;
+  }
+  return DSM_OK;
+}
+
+// Launch times of a timed pass, read once its events have completed (waits for them
+// if the caller has not synchronised yet).
+int pass_times(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
+  float ms = 0.0f;
+  if (ps.pending == 1) {
+    HIPCHK(hipEventSynchronize(e->lev[2 * ps.ns - 1]));
+    for (int k = 0; k < ps.ns; ++k) {
       HIPCHK(hipEventElapsedTime(&ms, e->lev[2 * k], e->lev[2 * k + 1]));
       rs->kmc_ms += ms;
       if (k == 0) rs->first_kmc_ms += ms;
     }
+  } else if (ps.pending == 2) {
+    HIPCHK(hipEventSynchronize(e->cev[3]));
+    HIPCHK(hipEventElapsedTime(&ms, e->cev[0], e->cev[3]));
+    rs->kmc_ms += ms;
+    HIPCHK(hipEventElapsedTime(&ms, e->cev[1], e->cev[2]));
+    rs->first_kmc_ms += ms;
   }
+  ps.pending = 0;
   return DSM_OK;
 }
 
 // One KMC pass over every chain of the given ensembles until each reaches its
 // ensemble's P.t_end (batched: persistent geometry-binned launches; plain: one jump
 // per chain per pass until no chain moves).  P must already be uploaded.
+// With `later`, a batched pass's launch times are left in *later for pass_times after
+// the caller's own synchronisation (no host wait between the pass and what follows it).
 int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_stop,
-             volatile const int32_t* run_flag, dsm_run_stats* rs) {
+             volatile const int32_t* run_flag, dsm_run_stats* rs, Pass* later = nullptr) {
   Pass ps;
   int rc = pass_setup(e, ids, use_ev_stop, ps);
   if (rc || ps.n0 == 0) return rc;
@@ -764,6 +780,8 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_s
     return DSM_OK;
   }
   if ((rc = pass_bins(e, ps, false, true, rs))) return rc;
+  if (later) *later = ps;
+  else if ((rc = pass_times(e, ps, rs))) return rc;
   if (run_flag && *run_flag == 0) return DSM_ABORTED;
   return DSM_OK;
 }
@@ -785,8 +803,21 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
   dsm_run_stats rs{};
   int rc = set_device(e);
   if (rc) return rc;
-  unsigned long long before[2];
-  if ((rc = chain_totals(e, ids, before))) return rc;
+  // jump and Z totals before (d_counters[0..1]) and after (d_counters[2..3]) the call,
+  // summed on the device: the "after" sums and their copy ride on the last window's
+  // synchronisation, so the totals cost no host round trip of their own
+  unsigned long long tot[4] = {0, 0, 0, 0};
+  bool have_tot = false;
+  HIPCHK(launch_fill_u32(e->d_counters, 0u, 8, e->st));
+  for (int id : ids)
+    HIPCHK(launch_sum_counters(e->ch.ev, e->ch.zsum, e->ens[id].chain0, e->ens[id].n, e->d_counters, e->st));
+  auto aborted = [&]() {
+    unsigned long long before[2] = {0, 0};
+    if (hipMemcpy(before, e->d_counters, sizeof(before), hipMemcpyDeviceToHost) == hipSuccess)
+      finish_stats(e, ids, before, &rs, T);
+    if (out) *out = rs;
+    return DSM_ABORTED;
+  };
   // per-ensemble sample targets
   std::vector<int64_t> k_target(ids.size(), 0);
   for (size_t q = 0; q < ids.size(); ++q) {
@@ -831,16 +862,14 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
       act.push_back(ids[q]);
     }
     if (act.empty()) break;
+    const bool last = std::all_of(reached.begin(), reached.end(), [](char r) { return r != 0; });
     if ((rc = upload_ens(e))) return rc;
     for (int id : act) {
       Ens& E = e->ens[id];
       if (E.P.k_end > E.P.k0)
         HIPCHK(launch_fill_u32(e->d_stage + E.P.stage_off, 0u, (E.P.k_end - E.P.k0) * E.n * E.ncomp, e->st));
     }
-    rc = kmc_pass(e, act, mode, false, run_flag, &rs);
-    if (rc == DSM_ABORTED) { finish_stats(e, ids, before, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
-    if (rc) return rc;
-    // consume the window's samples: every reduction is queued, then one synchronisation
+    // the window's sample consumers: level counters uploaded before the pass
     std::vector<int> cnt;
     std::vector<int64_t> cnt_off(act.size(), -1), row_off(act.size(), -1);
     int64_t flow_doubles = 0;
@@ -869,6 +898,11 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
     }
     if (!cnt.empty())
       HIPCHK(hipMemcpyAsync(e->d_corr_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, e->st));
+    Pass timed;
+    rc = kmc_pass(e, act, mode, false, run_flag, &rs, &timed);
+    if (rc == DSM_ABORTED) { pass_times(e, timed, &rs); return aborted(); }
+    if (rc) return rc;
+    // consume the window's samples: every reduction is queued, then one synchronisation
     for (size_t q = 0; q < act.size(); ++q) {
       Ens& E = e->ens[act[q]];
       const int rows = (int)(E.P.k_end - E.P.k0);
@@ -885,7 +919,15 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
     if (flow_doubles)
       HIPCHK(hipMemcpyAsync(flow_host.data(), e->d_rows, sizeof(double) * flow_doubles, hipMemcpyDeviceToHost,
                             e->st));
+    if (last) {
+      for (int id : ids)
+        HIPCHK(launch_sum_counters(e->ch.ev, e->ch.zsum, e->ens[id].chain0, e->ens[id].n, e->d_counters + 2,
+                                   e->st));
+      HIPCHK(hipMemcpyAsync(tot, e->d_counters, sizeof(tot), hipMemcpyDeviceToHost, e->st));
+      have_tot = true;
+    }
     HIPCHK(hipStreamSynchronize(e->st));
+    if ((rc = pass_times(e, timed, &rs))) return rc;
     for (size_t q = 0; q < act.size(); ++q) {
       Ens& E = e->ens[act[q]];
       const int64_t rows = E.P.k_end - E.P.k0;
@@ -906,9 +948,17 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
       const double frac = k_target[0] > 0 ? (double)E.k_cur / (double)k_target[0] : 1.0;
       *progress = (int32_t)(100.0 * frac);
     }
-    if (run_flag && *run_flag == 0) { finish_stats(e, ids, before, &rs, T); if (out) *out = rs; return DSM_ABORTED; }
+    if (run_flag && *run_flag == 0 && !last) return aborted();
   }
-  if ((rc = finish_stats(e, ids, before, &rs, T))) return rc;
+  if (have_tot) {
+    rs.events = tot[2] - tot[0];
+    rs.z_sum = tot[3] - tot[1];
+    rs.total_ms = T.ms();
+  } else {
+    unsigned long long before[2] = {0, 0};
+    HIPCHK(hipMemcpy(before, e->d_counters, sizeof(before), hipMemcpyDeviceToHost));
+    if ((rc = finish_stats(e, ids, before, &rs, T))) return rc;
+  }
   if (out) *out = rs;
   return DSM_OK;
 }
