@@ -574,10 +574,14 @@ struct Pass {
   int n_live = 0;           // chains of the pass
   int pending = 0;          // launch times recorded, not yet read: 1 sequential (ns launches), 2 concurrent
   int ns = 0;
+  bool range = false;       // one ensemble, chains first .. first+n_live-1, no bin-0 list written
+  int first = 0;
   KmcArgs a{};
 };
 
-int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pass& ps) {
+// `batched`: a batched pass (a one-ensemble sequential one reads its chains by range and
+// needs no bin-0 list)
+int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pass& ps, bool batched = false) {
   int n0 = 0, bin_max = 0, tab_max = 0, ext = 0;
   double z_est = 0.0, n_all = 0.0;
   for (int id : ids) {
@@ -597,11 +601,15 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   // 63 positions (the pair form then carries the bulk)
   ps.concurrent = bin_max >= 3 || (bin_max >= 1 && z_est >= 56.0 * n_all);
   if (n0 == 0) return DSM_OK;
-  HIPCHK(launch_fill_u32(e->d_work, 0xFFFFFFFFu, n0, e->st));
-  int off = 0;
-  for (int id : ids) {
-    HIPCHK(launch_iota(e->d_work + off, e->ens[id].chain0, e->ens[id].n, e->st));
-    off += (e->ens[id].n + 3) & ~3;
+  ps.range = batched && !ps.concurrent && ids.size() == 1;
+  ps.first = e->ens[ids[0]].chain0;
+  if (!ps.range) {
+    HIPCHK(launch_fill_u32(e->d_work, 0xFFFFFFFFu, n0, e->st));
+    int off = 0;
+    for (int id : ids) {
+      HIPCHK(launch_iota(e->d_work + off, e->ens[id].chain0, e->ens[id].n, e->st));
+      off += (e->ens[id].n + 3) & ~3;
+    }
   }
   ps.a = base_args(e);
   ps.a.use_ev_stop = use_ev_stop ? 1 : 0;
@@ -682,8 +690,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
 int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs) {
   if (!plain && ps.concurrent) return pass_bins_concurrent(e, ps, rs);
   int* cnt = e->d_cnt;
-  HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumLists, e->st));
-  HIPCHK(launch_fill_u32(cnt, (uint32_t)ps.n0, 1, e->st));
+  HIPCHK(launch_init_counts(cnt, 2 * kNumLists + 17, ps.n0, e->st));  // lists, queues, order scratch
   KmcArgs a = ps.a;
   int seq[kNumBins], ns = 0;
   seq[ns++] = 0;
@@ -700,8 +707,9 @@ int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs
   const bool zorder = !plain && (ps.n_ens == 1 || small);
   constexpr int kZList = kNumLists - 1;
   if (zorder)
-    HIPCHK(launch_z_order(e->d_work, ps.n0, e->ch.Z, e->ch.stall, cnt + 2 * kNumLists,
-                          e->d_work + (size_t)kZList * e->d_work_cap, cnt + kZList, e->st));
+    HIPCHK(launch_z_order(ps.range ? nullptr : e->d_work, ps.first, ps.range ? ps.n_live : ps.n0, e->ch.Z,
+                          e->ch.stall, cnt + 2 * kNumLists, e->d_work + (size_t)kZList * e->d_work_cap,
+                          cnt + kZList, e->st));
   if (small) {  // the compact ordered list feeds the wave form directly
     ns = 0;
     for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) seq[ns++] = b;
@@ -758,7 +766,7 @@ int pass_times(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
 int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_stop,
              volatile const int32_t* run_flag, dsm_run_stats* rs, Pass* later = nullptr) {
   Pass ps;
-  int rc = pass_setup(e, ids, use_ev_stop, ps);
+  int rc = pass_setup(e, ids, use_ev_stop, ps, mode != DSM_MODE_PLAIN);
   if (rc || ps.n0 == 0) return rc;
   if (mode == DSM_MODE_PLAIN) {
     const int batch = 16;

@@ -158,12 +158,14 @@ constexpr int kZBuckets = 17;
 
 // Block-local histograms in LDS, one global atomic per block and bucket (a per-element
 // global atomic on 17 counters serialises in L2).  Grid-stride over the elements.
-__global__ void __launch_bounds__(1024) zhist_kernel(const int* src, int n, const int* Z, const int* stall, int* hist) {
+// src == null: the chains first .. first+n-1 (one ensemble's range)
+__global__ void __launch_bounds__(1024) zhist_kernel(const int* src, int first, int n, const int* Z, const int* stall,
+                                                     int* hist) {
   __shared__ int lh[kZBuckets];
   if (threadIdx.x < kZBuckets) lh[threadIdx.x] = 0;
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int c = src[i];
+    const int c = src ? src[i] : first + i;
     if (c >= 0) atomicAdd(lh + zbucket(Z, stall, c), 1);
   }
   __syncthreads();
@@ -180,13 +182,13 @@ __global__ void zscan_kernel(int* hist, int* n_out) {
 
 // Each block reserves its range of every bucket with one atomic, then places its
 // elements with LDS atomics (order within a bucket is arbitrary).
-__global__ void __launch_bounds__(1024) zscatter_kernel(const int* src, int n, const int* Z, const int* stall,
-                                                        int* cursor, int* dst) {
+__global__ void __launch_bounds__(1024) zscatter_kernel(const int* src, int first, int n, const int* Z,
+                                                        const int* stall, int* cursor, int* dst) {
   __shared__ int lh[kZBuckets], base[kZBuckets];
   if (threadIdx.x < kZBuckets) lh[threadIdx.x] = 0;
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int c = src[i];
+    const int c = src ? src[i] : first + i;
     if (c >= 0) atomicAdd(lh + zbucket(Z, stall, c), 1);
   }
   __syncthreads();
@@ -196,12 +198,18 @@ __global__ void __launch_bounds__(1024) zscatter_kernel(const int* src, int n, c
   }
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int c = src[i];
+    const int c = src ? src[i] : first + i;
     if (c >= 0) {
       const int b = zbucket(Z, stall, c);
       dst[base[b] + atomicAdd(lh + b, 1)] = c;
     }
   }
+}
+
+// counts of a sequential pass: cnt[0] = n0, the rest (lists, queues, order scratch) zero
+__global__ void init_counts_kernel(int* cnt, int n, int n0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = i == 0 ? n0 : 0;
 }
 
 __global__ void iota_kernel(int* p, int start, int n) {
@@ -684,14 +692,18 @@ hipError_t launch_split_geoms(const int* src, int n, const int* Z, const int* st
   return hipGetLastError();
 }
 
-hipError_t launch_z_order(const int* src, int n, const int* Z, const int* stall, int* scratch17, int* dst,
+hipError_t launch_init_counts(int* cnt, int n, int n0, hipStream_t st) {
+  hipLaunchKernelGGL(init_counts_kernel, dim3((n + 255) / 256), dim3(256), 0, st, cnt, n, n0);
+  return hipGetLastError();
+}
+
+hipError_t launch_z_order(const int* src, int first, int n, const int* Z, const int* stall, int* scratch17, int* dst,
                           int* n_out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, (uint32_t*)scratch17, 0u, (int64_t)kZBuckets);
   const int blocks = n > 64 * 1024 * 4 ? 256 : (n + 4095) / 4096 + 1;  // a few elements per thread
-  hipLaunchKernelGGL(zhist_kernel, dim3(blocks), dim3(1024), 0, st, src, n, Z, stall, scratch17);
+  hipLaunchKernelGGL(zhist_kernel, dim3(blocks), dim3(1024), 0, st, src, first, n, Z, stall, scratch17);
   hipLaunchKernelGGL(zscan_kernel, dim3(1), dim3(64), 0, st, scratch17, n_out);
-  hipLaunchKernelGGL(zscatter_kernel, dim3(blocks), dim3(1024), 0, st, src, n, Z, stall, scratch17, dst);
+  hipLaunchKernelGGL(zscatter_kernel, dim3(blocks), dim3(1024), 0, st, src, first, n, Z, stall, scratch17, dst);
   return hipGetLastError();
 }
 

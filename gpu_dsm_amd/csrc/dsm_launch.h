@@ -31,9 +31,12 @@ hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const flo
                           hipStream_t st);
 hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_iota(int* p, int start, int n, hipStream_t st);
-// the chains of src (entries < 0 skipped) into dst in decreasing Z; *n_out = their count
-hipError_t launch_z_order(const int* src, int n, const int* Z, const int* stall, int* scratch17, int* dst,
+// the chains of src (entries < 0 skipped; src == null: chains first .. first+n-1) into
+// dst in decreasing Z; *n_out = their count.  scratch17 (17 ints) must be zero.
+hipError_t launch_z_order(const int* src, int first, int n, const int* Z, const int* stall, int* scratch17, int* dst,
                           int* n_out, hipStream_t st);
+// cnt[0] = n0, cnt[1..n) = 0
+hipError_t launch_init_counts(int* cnt, int n, int n0, hipStream_t st);
 hipError_t launch_split_geoms(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
                               int* counts, hipStream_t st);
 // out[0] += sum ev[chain0..chain0+n), out[1] += sum zsum[...]
