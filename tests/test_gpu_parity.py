@@ -112,6 +112,7 @@ def test_init_parity(engine, oracle, case):
 
 # ------------------------------------------------------------------ KMC jumps
 KMC_CASES = [
+    dict(n=1, nk=30, beta=1.0, cd=1, events=200),      # a single chain
     dict(n=300, nk=25, beta=1.0, cd=0, events=400),
     dict(n=200, nk=100, beta=1.0, cd=1, events=300),
     dict(n=127, nk=50, beta=0.5, cd=1, events=300),
