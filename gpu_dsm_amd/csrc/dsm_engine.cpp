@@ -42,8 +42,8 @@ int fail(int code, const char* fmt, ...) {
 constexpr int kNumBins = kNumGeoms;  // work bins = KMC chain geometries (capacity 64 << b)
 // Work lists (cap entries each).  Sequential pass: list b feeds bin b, overflow to b+1.
 // Concurrent pass: list 0 holds the pass's chains, lists 1..kNumBins the per-geometry
-// split (round 1), lists kNumBins+1.. the chains that outgrew their geometry in a round
-// (round 2: late list of bin b = kNumBins + b).
+// split (the first launches), lists kNumBins+1.. the chains that outgrew their geometry
+// there (the late launches: late list of bin b = kNumBins + b).
 constexpr int kNumLists = 2 * kNumBins + 1;
 constexpr int kSplitList = 1, kLateList = kNumBins;
 constexpr int kMaxNk = 2047;         // Z <= Nk must fit the largest geometry after a creation
@@ -651,7 +651,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   HIPCHK(launch_split_geoms(list(0), ps.n0, e->ch.Z, e->ch.stall, list(kSplitList), (int)cap, cnt + kSplitList,
                             e->st));
   HIPCHK(hipEventRecord(e->ev_split, e->st));
-  // round 1.  A process has four hardware queues on this device, so four streams: the
+  // first launches.  A process has four hardware queues on this device, so four streams: the
   // row form and the pair form on their own; the two long-chain geometries (few chains,
   // latency-bound runs) start at once on the other two, each followed by a wave-form
   // bulk geometry (G = 4 then G = 2, G = 8 then G = 1).  Measured on the polydisperse
@@ -671,7 +671,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   HIPCHK(launch_kmc(a, 0, false, grid(0), e->st));
   HIPCHK(hipEventRecord(e->cev[2], e->st));
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[k], 0));
-  // round 2: chains that outgrew their geometry, through the larger ones in order
+  // late launches: chains that outgrew their geometry, through the larger ones in order
   for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) {
     bind(kLateList + b, late_of(b));
     HIPCHK(launch_kmc(a, b, false, grid(b), e->st));
