@@ -7,8 +7,11 @@
  *   end rates ........ common/eq_ensemble_kernel.cu:340-394
  *   clock + pick ..... common/eq_ensemble_kernel.cu:395-419 (dt = 1/W, pr = W*u, linear scan)
  *   event effects .... common/eq_ensemble_kernel.cu:420-662, common/ensemble_kernel.cu:290-501
- *   deformation ...... common/ensemble_kernel.cu:105-110,135-146
+ *   deformation ...... common/ensemble_kernel.cu:105-110,135-146 (pinned in tests/test_oracle_flow.py:
+ *                      exact affine map from the clock's dt sequence, linear response,
+ *                      Green-Kubo, second-order N1)
  *   stress ........... common/ensemble_kernel.cu:504-538, common/eq_ensemble_kernel.cu:281-296
+ *                      (full tensor, Lpp and Z pinned by closed forms at rest)
  *   p_CD ............. common/pcd_tau.h:32-82, common/gpu_random.cu:76-99,143-148
  *   CD prefactor ..... common/ensemble.cu:136-150
  *   init ............. common/chain.cu:38-215 (same distributions; re-keyed on Philox)
