@@ -78,6 +78,8 @@ int fetch(dsm_engine* e, int ens, HostState& h) {
                        h.stall.data(), h.dt.data(), h.ns.data());
 }
 
+thread_local std::string g_io_err;
+
 // common/ensemble.cu:342-354, common/chain.cu:236-242
 bool save_conformations(const std::string& path, const HostState& h, double t_ref) {
   std::ofstream f(path, std::ios::out | std::ios::binary);
@@ -100,18 +102,26 @@ bool save_conformations(const std::string& path, const HostState& h, double t_re
   return f.good();
 }
 
-// common/ensemble.cu:486-504, common/chain.cu:244-250
+// common/ensemble.cu:486-504, common/chain.cu:244-250.  The head's unused float and the
+// rate slot of the last strand are read and ignored (the reference leaves them unset).
 int load_conformations(const std::string& path, HostState& h, int n_expected) {
   std::ifstream f(path, std::ios::in | std::ios::binary);
-  if (!f.is_open()) return DSM_ERR_IO;
+  if (!f.is_open()) { g_io_err = "cannot open " + path; return DSM_ERR_IO; }
   int32_t n = 0;
   f.read((char*)&n, 4);
-  if (!f) return DSM_ERR_IO;
-  if (n != n_expected) return DSM_ERR_MISMATCH;
+  if (!f) { g_io_err = path + ": no chain count"; return DSM_ERR_IO; }
+  if (n != n_expected) {
+    g_io_err = path + ": ensemble size mismatch (" + std::to_string(n) + " chains, " + std::to_string(n_expected) +
+               " expected)";
+    return DSM_ERR_MISMATCH;
+  }
   for (int i = 0; i < h.n; ++i) {
     ChainHead hd;
     f.read((char*)&hd, sizeof(hd));
-    if (!f || hd.Z < 1 || hd.Z > h.cap) return DSM_ERR_IO;
+    if (!f || hd.Z < 1 || hd.Z > h.cap) {
+      g_io_err = path + ": chain " + std::to_string(i) + (f ? ": Z out of range" : ": file ends inside its record");
+      return DSM_ERR_IO;
+    }
     h.Z[i] = hd.Z;
     h.t[i] = (double)hd.time;
     h.stall[i] = hd.stall_flag;
@@ -127,7 +137,7 @@ int load_conformations(const std::string& path, HostState& h, int n_expected) {
       f.read((char*)&r, 4);
       h.w[(size_t)i * h.cap + k] = (k < hd.Z - 1) ? r : 0.0f;
     }
-    if (!f) return DSM_ERR_IO;
+    if (!f) { g_io_err = path + ": chain " + std::to_string(i) + ": file ends inside its record"; return DSM_ERR_IO; }
   }
   return DSM_OK;
 }
@@ -191,6 +201,39 @@ void save_Q_distribution(const std::string& path, const HostState& h) {
 }
 
 }  // namespace
+
+extern "C" int dsm_save(dsm_engine* e, int32_t ens, const char* path, double t_ref) {
+  int32_t n = 0, cap = 0;
+  int rc = dsm_ensemble_info(e, ens, &n, &cap, nullptr);
+  if (rc) return rc;
+  if (!path) return DSM_ERR_ARG;
+  HostState h;
+  h.resize(n, cap);
+  if ((rc = fetch(e, ens, h))) return rc;
+  return save_conformations(path, h, t_ref) ? DSM_OK : DSM_ERR_IO;
+}
+
+extern "C" int dsm_load(dsm_engine* e, int32_t ens, const char* path) {
+  int32_t n = 0, cap = 0, mode = 0;
+  int rc = dsm_ensemble_info(e, ens, &n, &cap, &mode);
+  if (rc) return rc;
+  if (!path) return DSM_ERR_ARG;
+  HostState h;
+  h.resize(n, cap);
+  g_io_err.clear();
+  if ((rc = load_conformations(path, h, n))) return rc;
+  // sampling restarts at the loaded clocks: flow output 1 at delta, G(t) sample 0
+  for (int i = 0; i < n; ++i) h.ns[i] = mode == DSM_SAMPLE_FLOW ? 1 : 0;
+  rc = dsm_set_state(e, ens, cap, h.Z.data(), h.N.data(), h.Q.data(), h.w.data(), h.t.data(), nullptr,
+                     h.stall.data(), nullptr, h.ns.data());
+  if (rc == DSM_ERR_ARG) {
+    g_io_err = std::string(path) + ": invalid chain record (" + dsm_last_error() + ")";
+    return DSM_ERR_IO;
+  }
+  return rc;
+}
+
+extern "C" const char* dsm_load_error(void) { return g_io_err.c_str(); }
 
 extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char* savefile, const char* loadfile,
                            int device_ID, bool distr, volatile int* progress_bar, const dsm_main_options* opt) {
@@ -278,16 +321,11 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
   if (loadfile) {
     std::cout << "loading chain conformations from " << loadfile << "..";
     std::cout.flush();
-    rc = load_conformations(loadfile, h, in.N_cha);
+    rc = dsm_load(e, ens, loadfile);
     if (rc == DSM_ERR_MISMATCH) std::cout << "ensemble size mismatch\n";
+    else if (rc == DSM_ERR_IO) std::cout << "file error: " << g_io_err << "\n";
+    else if (rc) std::cout << "error: " << dsm_last_error() << "\n";
     if (rc) { dsm_destroy(e); return rc; }
-    for (int i = 0; i < h.n; ++i) h.ns[i] = flow ? 1 : 0;
-    if ((rc = dsm_set_state(e, ens, cap, h.Z.data(), h.N.data(), h.Q.data(), h.w.data(), h.t.data(), nullptr,
-                            h.stall.data(), nullptr, h.ns.data()))) {
-      std::cout << "error: " << dsm_last_error() << "\n";
-      dsm_destroy(e);
-      return rc;
-    }
     std::cout << "done.\n";
   } else {
     std::cout << "generating chain conformations on device..";
@@ -383,7 +421,7 @@ extern "C" int dsm_main_ex(volatile const bool* run_flag, int job_ID, const char
   }
   if (savefile) {
     std::cout << "saving chain conformations to " << savefile << "..";
-    if (!save_conformations(savefile, h, t_ref)) std::cout << "file error\n";
+    if (dsm_save(e, ens, savefile, t_ref) != DSM_OK) std::cout << "file error\n";
     std::cout << "done.\n";
   }
   if (distr) {

@@ -48,6 +48,8 @@ constexpr int kNumLists = 2 * kNumBins + 1;
 constexpr int kSplitList = 1, kLateList = kNumBins;
 constexpr int kMaxNk = 2047;         // Z <= Nk must fit the largest geometry after a creation
 constexpr int kDefaultWindow = 256;
+constexpr int kTotalsSlot = 4;       // d_counters: [0..3] advance_impl, [4..5] chain_totals
+constexpr int kNumCounters = 6;
 constexpr int kLdsTableFloats = 2816;  // per-wave LDS budget for rate tables (Nk <= ~200)
 
 // ---------------------------------------------------------------- p_CD (FP32 host, as the reference)
@@ -315,12 +317,26 @@ struct Ens {
   std::vector<int64_t> nins;
   // flow output rows (9 doubles per row)
   std::vector<double> flow_rows;
+  int tab_slot = -1, pd_slot = -1;  // table arena slots (rate tables, partner tables)
 };
 
 }  // namespace
 
+// Scheduling constants of the engine, set by dsm_set_option (and by the autotuner);
+// none of them changes a result, only how the passes are laid out on the device.
+struct EngineOptions {
+  int plain_jumps = 1;                   // DSM_OPT_PLAIN_JUMPS
+  int small_per_cu = 12;                 // DSM_OPT_SMALL_PER_CU
+  int split_margin_div = 8;              // DSM_OPT_SPLIT_MARGIN
+  int lds_table_floats = kLdsTableFloats;  // DSM_OPT_LDS_TABLE_FLOATS
+  int concurrent = -1;                   // DSM_OPT_CONCURRENT
+  int blocks_per_cu = 0;                 // DSM_OPT_BLOCKS_PER_CU
+  int window = kDefaultWindow;           // DSM_OPT_WINDOW
+};
+
 struct dsm_engine {
   int device = 0;
+  EngineOptions opt;
   hipStream_t st = nullptr;
   hipStream_t xs[4] = {};  // xs[1..3]: the other streams of a concurrent pass (xs[0] stays null; st is stream 0)
   hipEvent_t ev_split = nullptr, ev_bin[4] = {};
@@ -335,15 +351,23 @@ struct dsm_engine {
   std::vector<Ens> ens;
   EnsParams* d_ens = nullptr;
   int d_ens_cap = 0;
+  // Rate tables (kind 0) and polydisperse partner tables (kind 1) in one float arena.  A
+  // slot is shared by every ensemble whose parameters give the same tables and counts its
+  // users; a slot whose last user is removed keeps its contents (a later ensemble with
+  // the same key takes it back as is) until a table of the same kind and size needs room.
   std::vector<float> tabs;
   float* d_tabs = nullptr;
   size_t d_tabs_cap = 0;
-  struct TabKey {
-    int nmax, cd;
-    float beta, c_create;
-    int off;
+  struct TabSlot {
+    int kind = 0;
+    int nmax = 0, cd = 0, mode = 0;
+    float beta = 0.0f, c_create = 0.0f;
+    float gex[4] = {0, 0, 0, 0};
+    std::vector<float> nkF;  // melt partners: the quantile lengths
+    int64_t off = 0, size = 0;
+    int refs = 0;
   };
-  std::vector<TabKey> tab_keys;  // tables are shared by ensembles with equal parameters
+  std::vector<TabSlot> tab_slots;
   std::vector<int64_t> stage_off;  // per ensemble
   int64_t stage_used = 0;
   RangeAlloc chain_space, strand_space, stage_space;
@@ -361,6 +385,7 @@ struct dsm_engine {
   double* d_rows = nullptr;  // flow rows of one window, all ensembles
   int64_t d_rows_cap = 0;
   int* d_healthy = nullptr;
+  double* d_part = nullptr;  // per-block partial sums of dsm_stress
   hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> host_work;
 };
@@ -455,16 +480,65 @@ int upload_ens(dsm_engine* e) {
   return DSM_OK;
 }
 
-int upload_tabs(dsm_engine* e) {
+// floats [off, off+n) of the table arena to the device (all of it when the device copy grows)
+int upload_tabs(dsm_engine* e, int64_t off, int64_t n) {
   if (e->tabs.size() > e->d_tabs_cap) {
     if (e->d_tabs) HIPCHK(hipFree(e->d_tabs));
+    e->d_tabs = nullptr;
+    e->d_tabs_cap = 0;
     const size_t cap = std::max(e->tabs.size() * 2, (size_t)4096);
     int rc = dev_alloc(&e->d_tabs, cap);
     if (rc) return rc;
     e->d_tabs_cap = cap;
+    off = 0;
+    n = (int64_t)e->tabs.size();
   }
-  HIPCHK(hipMemcpy(e->d_tabs, e->tabs.data(), sizeof(float) * e->tabs.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->d_tabs + off, e->tabs.data() + off, sizeof(float) * n, hipMemcpyHostToDevice));
   return DSM_OK;
+}
+
+bool same_tables(const dsm_engine::TabSlot& a, const dsm_engine::TabSlot& b) {
+  if (a.kind != b.kind || a.beta != b.beta) return false;
+  if (a.kind == 0) return a.nmax == b.nmax && a.cd == b.cd && a.c_create == b.c_create;
+  if (a.mode != b.mode) return false;
+  if (a.mode == DSM_PARTNER_GEX) return std::equal(a.gex, a.gex + 4, b.gex);
+  return a.nkF == b.nkF;
+}
+
+// slot holding the tables of `key` (one more user); `make` fills them when no slot has
+// them: a free slot of the same kind and size is reused, else the arena grows
+template <typename Make>
+int take_tables(dsm_engine* e, const dsm_engine::TabSlot& key, Make make, int* slot) {
+  for (size_t i = 0; i < e->tab_slots.size(); ++i)
+    if (same_tables(e->tab_slots[i], key)) {
+      e->tab_slots[i].refs++;
+      *slot = (int)i;
+      return DSM_OK;
+    }
+  std::vector<float> data;
+  dsm_engine::TabSlot t = key;
+  make(data, t);
+  int idx = -1;
+  for (size_t i = 0; i < e->tab_slots.size(); ++i)
+    if (e->tab_slots[i].refs == 0 && e->tab_slots[i].kind == key.kind && e->tab_slots[i].size == (int64_t)data.size()) {
+      idx = (int)i;
+      break;
+    }
+  if (idx < 0) {
+    if (e->tabs.size() + data.size() > (size_t)INT32_MAX) return fail(DSM_ERR_MEMORY, "table arena full");
+    t.off = (int64_t)e->tabs.size();
+    e->tabs.resize(e->tabs.size() + data.size());
+    e->tab_slots.push_back(t);
+    idx = (int)e->tab_slots.size() - 1;
+  } else {
+    t.off = e->tab_slots[idx].off;
+  }
+  t.size = (int64_t)data.size();
+  t.refs = 1;
+  std::copy(data.begin(), data.end(), e->tabs.begin() + t.off);
+  e->tab_slots[idx] = t;
+  *slot = idx;
+  return upload_tabs(e, t.off, t.size);
 }
 
 int ensure_stage(dsm_engine* e) {
@@ -538,6 +612,7 @@ struct Timer {
 
 int device_blocks(dsm_engine* e, int geom, bool plain, int lds_floats, bool ext) {
   int nb = kmc_max_blocks_per_cu(geom, plain, lds_floats, ext);
+  if (e->opt.blocks_per_cu > 0) nb = std::min(nb, e->opt.blocks_per_cu);
   if (nb <= 0) nb = 1;
   return nb * e->n_cu;
 }
@@ -553,12 +628,14 @@ KmcArgs base_args(dsm_engine* e) {
   return a;
 }
 
-// sums of the per-chain jump counters and Z sums over the given ensembles
+// sums of the per-chain jump counters and Z sums over the given ensembles, in counter
+// slots 4..5 (slots 0..3 hold advance_impl's totals at the start and end of a call)
 int chain_totals(dsm_engine* e, const std::vector<int>& ids, unsigned long long out[2]) {
-  HIPCHK(launch_fill_u32(e->d_counters, 0u, 4, e->st));
+  unsigned long long* slot = e->d_counters + kTotalsSlot;
+  HIPCHK(launch_fill_u32(slot, 0u, 4, e->st));
   for (int id : ids)
-    HIPCHK(launch_sum_counters(e->ch.ev, e->ch.zsum, e->ens[id].chain0, e->ens[id].n, e->d_counters, e->st));
-  HIPCHK(hipMemcpyAsync(out, e->d_counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st));
+    HIPCHK(launch_sum_counters(e->ch.ev, e->ch.zsum, e->ens[id].chain0, e->ens[id].n, slot, e->st));
+  HIPCHK(hipMemcpyAsync(out, slot, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
   return DSM_OK;
 }
@@ -599,7 +676,8 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   // geometries side by side when chains beyond the first wave-form geometry can occur
   // (their runs are long and few) or when many chains are expected above the row form's
   // 63 positions (the pair form then carries the bulk)
-  ps.concurrent = bin_max >= 3 || (bin_max >= 1 && z_est >= 56.0 * n_all);
+  ps.concurrent = e->opt.concurrent >= 0 ? (e->opt.concurrent > 0 && bin_max >= 1)
+                                         : (bin_max >= 3 || (bin_max >= 1 && z_est >= 56.0 * n_all));
   if (n0 == 0) return DSM_OK;
   ps.range = batched && !ps.concurrent && ids.size() == 1;
   ps.first = e->ens[ids[0]].chain0;
@@ -614,8 +692,9 @@ int pass_setup(dsm_engine* e, const std::vector<int>& ids, bool use_ev_stop, Pas
   ps.a = base_args(e);
   ps.a.use_ev_stop = use_ev_stop ? 1 : 0;
   // rate tables go to LDS (one copy per wave) when they fit the per-wave budget
-  ps.a.lds_floats = tab_max <= kLdsTableFloats ? tab_max : 0;
+  ps.a.lds_floats = tab_max <= e->opt.lds_table_floats ? tab_max : 0;
   ps.a.ext = ext;
+  ps.a.plain_jumps = std::max(1, e->opt.plain_jumps);
   return DSM_OK;
 }
 
@@ -649,7 +728,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   HIPCHK(hipEventRecord(e->cev[0], e->st));
   HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumLists, e->st));
   HIPCHK(launch_split_geoms(list(0), ps.n0, e->ch.Z, e->ch.stall, list(kSplitList), (int)cap, cnt + kSplitList,
-                            e->st));
+                            e->opt.split_margin_div, e->st));
   HIPCHK(hipEventRecord(e->ev_split, e->st));
   // first launches.  A process has four hardware queues on this device, so four streams: the
   // row form and the pair form on their own; the two long-chain geometries (few chains,
@@ -703,7 +782,7 @@ int pass_bins(dsm_engine* e, Pass& ps, bool plain, bool timed, dsm_run_stats* rs
   // -- is latency-bound: it runs in the wave form, whose per-jump latency is the lowest
   // (1.79 us per jump against 2.84 us per row-form step of four chains).  Measured:
   // 1000 chains of Nk 25 +15%, of Nk 100 +92%; 4096 chains of Nk 25 -14%.
-  const bool small = !plain && ps.n_live <= 12 * e->n_cu;
+  const bool small = !plain && ps.n_live <= e->opt.small_per_cu * e->n_cu;
   const bool zorder = !plain && (ps.n_ens == 1 || small);
   constexpr int kZList = kNumLists - 1;
   if (zorder)
@@ -763,8 +842,10 @@ int pass_times(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
 // per chain per pass until no chain moves).  P must already be uploaded.
 // With `later`, a batched pass's launch times are left in *later for pass_times after
 // the caller's own synchronisation (no host wait between the pass and what follows it).
-int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_stop,
-             volatile const int32_t* run_flag, dsm_run_stats* rs, Pass* later = nullptr) {
+// The run flag is not polled here: a pass always completes its window, whose samples
+// the caller then consumes, so an aborted call leaves a state that continues exactly.
+int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_stop, dsm_run_stats* rs,
+             Pass* later = nullptr) {
   Pass ps;
   int rc = pass_setup(e, ids, use_ev_stop, ps, mode != DSM_MODE_PLAIN);
   if (rc || ps.n0 == 0) return rc;
@@ -783,14 +864,12 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_s
       rs->kmc_ms += ms;
       rs->first_kmc_ms += ms;
       if (after[0] == before[0]) break;
-      if (run_flag && *run_flag == 0) return DSM_ABORTED;
     }
     return DSM_OK;
   }
   if ((rc = pass_bins(e, ps, false, true, rs))) return rc;
   if (later) *later = ps;
   else if ((rc = pass_times(e, ps, rs))) return rc;
-  if (run_flag && *run_flag == 0) return DSM_ABORTED;
   return DSM_OK;
 }
 
@@ -907,9 +986,7 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
     if (!cnt.empty())
       HIPCHK(hipMemcpyAsync(e->d_corr_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, e->st));
     Pass timed;
-    rc = kmc_pass(e, act, mode, false, run_flag, &rs, &timed);
-    if (rc == DSM_ABORTED) { pass_times(e, timed, &rs); return aborted(); }
-    if (rc) return rc;
+    if ((rc = kmc_pass(e, act, mode, false, &rs, &timed))) return rc;
     // consume the window's samples: every reduction is queued, then one synchronisation
     for (size_t q = 0; q < act.size(); ++q) {
       Ens& E = e->ens[act[q]];
@@ -920,8 +997,7 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
                                   E.s, rows, e->d_corr_cnt + cnt_off[q], E.P.sample_mode == kSampleMsd ? 1 : 0,
                                   e->st));
       else
-        HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, e->ch.stall, E.chain0, E.n, rows,
-                                  e->d_rows + row_off[q], e->st));
+        HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, E.n, rows, e->d_rows + row_off[q], e->st));
     }
     std::vector<double> flow_host((size_t)flow_doubles);
     if (flow_doubles)
@@ -1012,8 +1088,8 @@ int dsm_create(int32_t device_id, dsm_engine** out) {
   HIPCHK(hipEventCreateWithFlags(&e->ev_split, hipEventDisableTiming));
   for (auto& ev : e->ev_bin) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   int rc;
-  if ((rc = dev_alloc(&e->d_cnt, 2 * kNumLists + 32)) || (rc = dev_alloc(&e->d_counters, 4)) ||
-      (rc = dev_alloc(&e->d_healthy, 1))) {
+  if ((rc = dev_alloc(&e->d_cnt, 2 * kNumLists + 32)) || (rc = dev_alloc(&e->d_counters, kNumCounters)) ||
+      (rc = dev_alloc(&e->d_healthy, 1)) || (rc = dev_alloc(&e->d_part, (size_t)kStressBlocks * 9))) {
     dsm_destroy(e);
     return rc;
   }
@@ -1041,7 +1117,7 @@ int dsm_destroy(dsm_engine* e) {
   ChainArrays& c = e->ch;
   void* ptrs[] = {c.Z, c.nk, c.ens, c.cap, c.stall, c.cid, c.local, c.base, c.t, c.ev, c.dt_prev, c.next_sample, c.ev_stop, c.zsum, c.R1,
                   c.Qx, c.Qy, c.Qz, c.N, c.w, e->d_ens, e->d_tabs, e->d_stage, e->d_work, e->d_cnt, e->d_counters,
-                  e->d_corr_cnt, e->d_red, e->d_rows, e->d_healthy};
+                  e->d_corr_cnt, e->d_red, e->d_rows, e->d_healthy, e->d_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& ev : e->evs)
@@ -1088,15 +1164,30 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   const int nk_ref = d->nk_per_chain ? nmax : d->nk;
   Pcd pc;
   pcd_init(pc, d->beta, (float)nk_ref);
-  // polydisperse CD background: partner quantiles and their sampler constants
-  std::vector<float> pdt;
+  // polydisperse CD background: partner quantiles and their sampler constants, shared by
+  // ensembles with the same background (GEX parameters, or the same melt quantiles)
   float c_partner = 0.0f;
   if (d->partner_mode != DSM_PARTNER_OWN) {
-    std::vector<float> nkF;
-    if ((d->partner_mode != DSM_PARTNER_GEX && d->partner_mode != DSM_PARTNER_MELT) ||
-        !partner_quantiles(d->partner_mode, d->gex, E.nk, nkF))
-      return fail(DSM_ERR_ARG, "bad partner_mode or GEX parameters");
-    c_partner = partner_tables(nkF, d->beta, pdt);
+    if (d->partner_mode != DSM_PARTNER_GEX && d->partner_mode != DSM_PARTNER_MELT)
+      return fail(DSM_ERR_ARG, "bad partner_mode");
+    dsm_engine::TabSlot key;
+    key.kind = 1;
+    key.mode = d->partner_mode;
+    key.beta = d->beta;
+    if (d->partner_mode == DSM_PARTNER_GEX) {
+      std::copy(d->gex, d->gex + 4, key.gex);
+    } else if (!partner_quantiles(DSM_PARTNER_MELT, d->gex, E.nk, key.nkF)) {
+      return fail(DSM_ERR_ARG, "bad partner lengths");
+    }
+    bool ok = true;
+    rc = take_tables(e, key, [&](std::vector<float>& data, dsm_engine::TabSlot& t) {
+      std::vector<float> nkF = t.nkF;
+      if (t.mode == DSM_PARTNER_GEX) ok = partner_quantiles(DSM_PARTNER_GEX, d->gex, E.nk, nkF);
+      if (ok) t.c_create = partner_tables(nkF, d->beta, data);
+    }, &E.pd_slot);
+    if (!ok) return fail(DSM_ERR_ARG, "bad GEX parameters");
+    if (rc) return rc;
+    c_partner = e->tab_slots[E.pd_slot].c_create;
   }
   EnsParams& P = E.P;
   P.beta = d->beta;
@@ -1129,27 +1220,25 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   P.k0 = 0;
   P.k_end = 0;
   // tables
-  P.tab_off = -1;
-  for (const auto& k : e->tab_keys)
-    if (k.nmax == nmax && k.cd == P.cd && k.beta == d->beta && k.c_create == P.c_create) P.tab_off = k.off;
-  if (P.tab_off < 0) {
-    std::vector<float> tb;
-    build_tables(tb, nmax, d->beta, P.cd, P.c_create);
-    P.tab_off = (int)e->tabs.size();
-    e->tabs.insert(e->tabs.end(), tb.begin(), tb.end());
-    e->tab_keys.push_back({nmax, P.cd, d->beta, P.c_create, P.tab_off});
-    if ((rc = upload_tabs(e))) return rc;
+  {
+    dsm_engine::TabSlot key;
+    key.kind = 0;
+    key.nmax = nmax;
+    key.cd = P.cd;
+    key.beta = d->beta;
+    key.c_create = P.c_create;
+    rc = take_tables(e, key, [&](std::vector<float>& data, dsm_engine::TabSlot&) {
+      build_tables(data, nmax, d->beta, P.cd, P.c_create);
+    }, &E.tab_slot);
+    if (rc) return rc;
   }
+  P.tab_off = (int)e->tab_slots[E.tab_slot].off;
   P.fr_off = P.tab_off + kNumTables * (nmax + 2);
-  if (P.partner) {
-    P.pd_off = (int)e->tabs.size();
-    e->tabs.insert(e->tabs.end(), pdt.begin(), pdt.end());
-    if ((rc = upload_tabs(e))) return rc;
-  }
+  if (P.partner) P.pd_off = (int)e->tab_slots[E.pd_slot].off;
   // sampling
   E.ncomp = (d->sample_mode == DSM_SAMPLE_GT || d->sample_mode == DSM_SAMPLE_MSD) ? 3
             : d->sample_mode == DSM_SAMPLE_FLOW ? 8 : 0;
-  E.window = d->window > 0 ? d->window : kDefaultWindow;
+  E.window = d->window > 0 ? d->window : e->opt.window;
   E.k_cur = d->sample_mode == DSM_SAMPLE_FLOW ? 1 : 0;
   E.p = d->corr_p > 0 ? d->corr_p : 64;
   E.m = d->corr_m > 0 ? d->corr_m : 8;
@@ -1208,6 +1297,9 @@ int dsm_remove_ensemble(dsm_engine* e, int32_t ens) {
   E->flow_rows.clear();
   E->flow_rows.shrink_to_fit();
   // its chains, strand slots and staging rows go back to the engine's free ranges
+  if (E->tab_slot >= 0) e->tab_slots[E->tab_slot].refs--;
+  if (E->pd_slot >= 0) e->tab_slots[E->pd_slot].refs--;
+  E->tab_slot = E->pd_slot = -1;
   e->chain_space.give(E->chain0, E->n);
   e->strand_space.give(E->strand0, E->nstrand);
   e->stage_space.give(E->P.stage_off, E->stage_floats);
@@ -1232,6 +1324,15 @@ int dsm_state_cap(dsm_engine* e, int32_t ens, int32_t* cap) {
   Ens* E = get_ens(e, ens);
   if (!E || !cap) return fail(DSM_ERR_ARG, "bad ensemble or pointer");
   *cap = E->nmax;
+  return DSM_OK;
+}
+
+int dsm_ensemble_info(dsm_engine* e, int32_t ens, int32_t* n_chains, int32_t* cap, int32_t* sample_mode) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  if (n_chains) *n_chains = E->n;
+  if (cap) *cap = E->nmax;
+  if (sample_mode) *sample_mode = E->P.sample_mode;
   return DSM_OK;
 }
 
@@ -1374,10 +1475,13 @@ int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mod
   E->P.k_end = E->k_cur;  // sampling suspended
   if ((rc = upload_ens(e))) return rc;
   if (mode == DSM_MODE_PLAIN) {
+    // every chain stops at its jump target; each launch makes up to plain_jumps of them
+    HIPCHK(launch_set_ev_stop(e->ch.ev, e->ch.ev_stop, E->chain0, E->n, (uint64_t)n_events, e->st));
     Pass ps;
-    if ((rc = pass_setup(e, {ens}, false, ps))) return rc;
+    if ((rc = pass_setup(e, {ens}, true, ps))) return rc;
+    const int64_t launches = (n_events + ps.a.plain_jumps - 1) / ps.a.plain_jumps;
     HIPCHK(hipEventRecord(e->evs[2], e->st));
-    for (int64_t k = 0; k < n_events; ++k)
+    for (int64_t k = 0; k < launches; ++k)
       if ((rc = pass_bins(e, ps, true, false, &rs))) return rc;
     HIPCHK(hipEventRecord(e->evs[3], e->st));
     HIPCHK(hipEventSynchronize(e->evs[3]));
@@ -1387,7 +1491,7 @@ int dsm_advance_events(dsm_engine* e, int32_t ens, int64_t n_events, int32_t mod
     rs.first_kmc_ms = ms;
   } else {
     HIPCHK(launch_set_ev_stop(e->ch.ev, e->ch.ev_stop, E->chain0, E->n, (uint64_t)n_events, e->st));
-    if ((rc = kmc_pass(e, {ens}, mode, true, nullptr, &rs))) return rc;
+    if ((rc = kmc_pass(e, {ens}, mode, true, &rs))) return rc;
   }
   if ((rc = finish_stats(e, {ens}, before, &rs, T))) return rc;
   if (stats) *stats = rs;
@@ -1406,15 +1510,87 @@ int dsm_flow_stress(dsm_engine* e, int32_t ens, int64_t first_row, int64_t max_r
   return DSM_OK;
 }
 
+int dsm_stress(dsm_engine* e, int32_t ens, double* out, int32_t* healthy) {
+  Ens* E = get_ens(e, ens);
+  if (!E || !out) return fail(DSM_ERR_ARG, "bad ensemble or pointer");
+  int rc = set_device(e);
+  if (rc) return rc;
+  HIPCHK(launch_chain_stress(e->ch, E->chain0, E->n, E->P.kappa, e->d_part, e->st));
+  std::vector<double> part((size_t)kStressBlocks * 9);
+  HIPCHK(hipMemcpyAsync(part.data(), e->d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  double sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < kStressBlocks; ++b)
+    for (int q = 0; q < 9; ++q) sum[q] += part[(size_t)b * 9 + q];
+  for (int q = 0; q < 8; ++q) out[q] = sum[8] > 0.0 ? sum[q] / sum[8] : 0.0;
+  if (healthy) *healthy = (int32_t)sum[8];
+  return DSM_OK;
+}
+
+int dsm_set_option(dsm_engine* e, int32_t key, int32_t value) {
+  if (!e) return fail(DSM_ERR_ARG, "null engine");
+  EngineOptions& o = e->opt;
+  switch (key) {
+    case DSM_OPT_PLAIN_JUMPS: if (value < 1) break; o.plain_jumps = value; return DSM_OK;
+    case DSM_OPT_SMALL_PER_CU: if (value < 0) break; o.small_per_cu = value; return DSM_OK;
+    case DSM_OPT_SPLIT_MARGIN: if (value < 0) break; o.split_margin_div = value; return DSM_OK;
+    case DSM_OPT_LDS_TABLE_FLOATS: if (value < 0) break; o.lds_table_floats = value; return DSM_OK;
+    case DSM_OPT_CONCURRENT: if (value < -1 || value > 1) break; o.concurrent = value; return DSM_OK;
+    case DSM_OPT_BLOCKS_PER_CU: if (value < 0) break; o.blocks_per_cu = value; return DSM_OK;
+    case DSM_OPT_WINDOW: if (value < 1) break; o.window = value; return DSM_OK;
+    default: return fail(DSM_ERR_ARG, "unknown option %d", key);
+  }
+  return fail(DSM_ERR_ARG, "option %d: value %d out of range", key, value);
+}
+
+int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value) {
+  if (!e || !value) return fail(DSM_ERR_ARG, "null argument");
+  const EngineOptions& o = e->opt;
+  switch (key) {
+    case DSM_OPT_PLAIN_JUMPS: *value = o.plain_jumps; return DSM_OK;
+    case DSM_OPT_SMALL_PER_CU: *value = o.small_per_cu; return DSM_OK;
+    case DSM_OPT_SPLIT_MARGIN: *value = o.split_margin_div; return DSM_OK;
+    case DSM_OPT_LDS_TABLE_FLOATS: *value = o.lds_table_floats; return DSM_OK;
+    case DSM_OPT_CONCURRENT: *value = o.concurrent; return DSM_OK;
+    case DSM_OPT_BLOCKS_PER_CU: *value = o.blocks_per_cu; return DSM_OK;
+    case DSM_OPT_WINDOW: *value = o.window; return DSM_OK;
+    default: return fail(DSM_ERR_ARG, "unknown option %d", key);
+  }
+}
+
+int dsm_check_state(dsm_engine* e, int32_t ens, double t_min, int64_t* out) {
+  Ens* E = get_ens(e, ens);
+  if (!E || !out) return fail(DSM_ERR_ARG, "bad ensemble or pointer");
+  int rc = set_device(e);
+  if (rc) return rc;
+  unsigned long long* cnt = (unsigned long long*)e->d_part;  // the stress partials' space, 7 counters
+  HIPCHK(launch_fill_u32(cnt, 0u, 14, e->st));
+  HIPCHK(launch_check_state(e->ch, E->chain0, E->n, t_min, cnt, e->st));
+  unsigned long long h[7];
+  HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  for (int q = 0; q < 7; ++q) out[q] = (int64_t)h[q];
+  return DSM_OK;
+}
+
 int dsm_correlator(dsm_engine* e, int32_t ens, int32_t max_pts, double* lag, double* g, int32_t* npts) {
+  Ens* E = get_ens(e, ens);
+  if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
+  return dsm_correlator_range(e, ens, 0, E->n, max_pts, lag, g, npts);
+}
+
+int dsm_correlator_range(dsm_engine* e, int32_t ens, int32_t first, int32_t count, int32_t max_pts, double* lag,
+                         double* g, int32_t* npts) {
   Ens* E = get_ens(e, ens);
   if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
   if (!eq_sampling(E->P.sample_mode) || !E->dC)
     return fail(DSM_ERR_STATE, "ensemble %d has no G(t) or MSD(t) correlator", ens);
+  if (first < 0 || count < 1 || first > E->n - count)
+    return fail(DSM_ERR_ARG, "chain range %d+%d outside 0..%d", first, count, E->n);
   int rc = set_device(e);
   if (rc) return rc;
   const int s = E->s, p = E->p, m = E->m;
-  HIPCHK(launch_corr_reduce(E->dC, e->ch.stall, E->chain0, E->n, s * p, e->d_red, e->d_healthy, e->st));
+  HIPCHK(launch_corr_reduce(E->dC, E->n, e->ch.stall, E->chain0, first, count, s * p, e->d_red, e->d_healthy, e->st));
   std::vector<double> sums(s * p);
   int healthy = 0;
   HIPCHK(hipMemcpyAsync(sums.data(), e->d_red, sizeof(double) * s * p, hipMemcpyDeviceToHost, e->st));

@@ -122,8 +122,10 @@ __global__ void set_ev_stop_kernel(const uint64_t* ev, uint64_t* ev_stop, int ch
 // chains go to geometry 0, which skips them.  Lists are cap entries apart, counts[b]
 // their lengths.
 // One atomic per wave and geometry: the lanes of a geometry get consecutive slots.
+// margin_div > 0: a chain within capacity/margin_div of a wave-form geometry's capacity
+// starts in the next one (it would likely outgrow it during the pass)
 __global__ void split_geoms_kernel(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
-                                   int* counts) {
+                                   int* counts, int margin_div) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = i < n ? src[i] : -1;
   int b = -1;
@@ -131,7 +133,9 @@ __global__ void split_geoms_kernel(const int* src, int n, const int* Z, const in
     const int z = Z[c];
     b = 0;
     if (stall[c] == 0)
-      while (b + 1 < kNumGeoms && z >= kmc_geom_capacity(b) - (b >= 2 ? kmc_geom_capacity(b) / 8 : 0)) ++b;
+      while (b + 1 < kNumGeoms &&
+             z >= kmc_geom_capacity(b) - (b >= 2 && margin_div > 0 ? kmc_geom_capacity(b) / margin_div : 0))
+        ++b;
   }
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -220,8 +224,12 @@ __global__ void iota_kernel(int* p, int start, int n) {
 // flow stress: per (sample, ensemble) FP64 sums of the 8 components over healthy
 // chains (stall == 0, finite, recorded); out[...*9 + 8] = chain count.
 // (common/ensemble_call_block.cu:452-505 averages in FP32 on the host.)
-__global__ void __launch_bounds__(256) flow_reduce_kernel(const float* stage, const int* stall, int chain0,
-                                                            int nchains, int nrows, double* out) {
+// A row of a chain counts when the chain recorded it (Z column >= 1: staging rows start
+// at zero and a chain that has stalled records nothing after the stall) and it holds no
+// NaN, so a chain that stalls mid-window keeps the rows it recorded before, as the
+// reference excludes a stalled chain only from the outputs after its stall
+// (common/ensemble_call_block.cu:471-491, per output step).
+__global__ void __launch_bounds__(256) flow_reduce_kernel(const float* stage, int nchains, int nrows, double* out) {
   const int row = blockIdx.x;
   if (row >= nrows) return;
   __shared__ double red[9][256];
@@ -230,7 +238,7 @@ __global__ void __launch_bounds__(256) flow_reduce_kernel(const float* stage, co
   for (int q = 0; q < 9; ++q) acc[q] = 0.0;
   for (int c = threadIdx.x; c < nchains; c += blockDim.x) {
     const float* o = stage + ((int64_t)row * nchains + c) * 8;
-    bool ok = stall[chain0 + c] == 0 && o[7] > 0.0f;
+    bool ok = o[7] > 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) ok = ok && (o[q] == o[q]);
     if (ok) {
@@ -487,17 +495,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) c
   A[0] = ax; A[nch] = ay; A[2 * nch] = az;
 }
 
-// G(level, lag) = sum over healthy chains of C / (3 * n_pairs * n_healthy)
-__global__ void __launch_bounds__(256) corr_reduce_kernel(const double* C, const int* stall, int chain0, int nch,
-                                                            int npts, double* out_sum, int* out_healthy) {
+// G(level, lag) = sum over healthy chains of C / (3 * n_pairs * n_healthy); the chains
+// are first .. first+count-1 of an ensemble of ld chains (C is [point][ld])
+__global__ void __launch_bounds__(256) corr_reduce_kernel(const double* C, int ld, const int* stall, int chain0,
+                                                            int first, int count, int npts, double* out_sum,
+                                                            int* out_healthy) {
   const int pt = blockIdx.x;
   if (pt >= npts) return;
   __shared__ double red[256];
   __shared__ int cnt[256];
   double acc = 0.0;
   int h = 0;
-  for (int c = threadIdx.x; c < nch; c += blockDim.x) {
-    if (stall[chain0 + c] == 0) { acc += C[(int64_t)pt * nch + c]; ++h; }
+  for (int c = first + threadIdx.x; c < first + count; c += blockDim.x) {
+    if (stall[chain0 + c] == 0) { acc += C[(int64_t)pt * ld + c]; ++h; }
   }
   red[threadIdx.x] = acc;
   cnt[threadIdx.x] = h;
@@ -507,6 +517,102 @@ __global__ void __launch_bounds__(256) corr_reduce_kernel(const double* C, const
     __syncthreads();
   }
   if (threadIdx.x == 0) { out_sum[pt] = red[0]; if (pt == 0) *out_healthy = cnt[0]; }
+}
+
+// ------------------------------------------------------------------ ensemble stress now
+// Stress of the current state (common/ensemble_kernel.cu:504-538 per chain, averaged by
+// common/ensemble.cu:210-221): per chain sigma_ab = -sum_i 3 Q_ia Q_ib / N_i, Lpp =
+// sum_i |Q_i| and Z, in FP64 from the stored FP32 strands, after the deformation still
+// pending from the chain's last jump (dt_prev; the reference applies it with olddt).
+// Chains that have stalled are left out.  A fixed grid of kStressBlocks blocks strides
+// over the chains; block b writes its partial sums to part[9b .. 9b+8] (xx yy zz xy yz zx
+// Lpp Z count), which the host adds in block order, so the result does not depend on
+// timing.
+__global__ void __launch_bounds__(256) chain_stress_kernel(ChainArrays ch, int chain0, int n, KappaArg K,
+                                                           double* part) {
+  __shared__ double red[9][256];
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = chain0 + i;
+    if (ch.stall[c] != 0) continue;
+    const int Z = ch.Z[c];
+    const int64_t b = ch.base[c];
+    const double dt = (double)ch.dt_prev[c];
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < Z; ++k) {
+      double x = ch.Qx[b + k], y = ch.Qy[b + k], z = ch.Qz[b + k];
+      if (dt != 0.0) {
+        const double x1 = x + dt * ((double)K.k[0] * x + (double)K.k[1] * y + (double)K.k[2] * z);
+        const double y1 = y + dt * ((double)K.k[3] * x + (double)K.k[4] * y + (double)K.k[5] * z);
+        const double z1 = z + dt * ((double)K.k[6] * x + (double)K.k[7] * y + (double)K.k[8] * z);
+        x = x1; y = y1; z = z1;
+      }
+      const double t3 = 3.0 / (double)ch.N[b + k];
+      s[0] -= t3 * x * x; s[1] -= t3 * y * y; s[2] -= t3 * z * z;
+      s[3] -= t3 * x * y; s[4] -= t3 * y * z; s[5] -= t3 * x * z;
+      s[6] += sqrt(x * x + y * y + z * z);
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) acc[q] += s[q];
+    acc[7] += (double)Z;
+    acc[8] += 1.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) part[blockIdx.x * 9 + q] = red[q][0];
+  }
+}
+
+// ------------------------------------------------------------------ state invariants
+// Counts, over chains chain0 .. chain0+n-1, the violations of the model's invariants
+// (App. A.1 of the chain state): out[0] Z outside 1..Nk, out[1] a strand with N < 1,
+// out[2] sum of N != Nk, out[3] an end strand (index 0 or Z-1, Z > 1) with Q != 0,
+// out[4] a non-finite Q or CD rate, out[5] stalled chains, out[6] chains whose clock is
+// below t_min.  The counters (7 x uint64) must be zero on entry.
+__global__ void __launch_bounds__(256) check_state_kernel(ChainArrays ch, int chain0, int n, double t_min,
+                                                          unsigned long long* out) {
+  unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = chain0 + i;
+    const int Z = ch.Z[c], nk = ch.nk[c];
+    if (ch.stall[c] != 0) v[5]++;
+    if (!(ch.t[c] >= t_min)) v[6]++;
+    if (Z < 1 || Z > nk || Z > ch.cap[c]) { v[0]++; continue; }
+    const int64_t b = ch.base[c];
+    int64_t sum = 0;
+    bool bad_n = false, bad_f = false;
+    for (int k = 0; k < Z; ++k) {
+      const int N = ch.N[b + k];
+      bad_n = bad_n || N < 1;
+      sum += N;
+      const float x = ch.Qx[b + k], y = ch.Qy[b + k], z = ch.Qz[b + k];
+      bad_f = bad_f || !isfinite(x) || !isfinite(y) || !isfinite(z) || (k < Z - 1 && !isfinite(ch.w[b + k]));
+    }
+    if (bad_n) v[1]++;
+    if (sum != nk) v[2]++;
+    if (Z > 1) {
+      const int64_t e0 = b, e1 = b + Z - 1;
+      if (ch.Qx[e0] != 0.f || ch.Qy[e0] != 0.f || ch.Qz[e0] != 0.f || ch.Qx[e1] != 0.f || ch.Qy[e1] != 0.f ||
+          ch.Qz[e1] != 0.f)
+        v[3]++;
+    }
+    if (bad_f) v[4]++;
+  }
+#pragma unroll
+  for (int q = 0; q < 7; ++q)
+    if (v[q]) atomicAdd(out + q, v[q]);
 }
 
 // ------------------------------------------------------------------ copy probe (bandwidth)
@@ -685,10 +791,10 @@ hipError_t launch_sum_counters(const uint64_t* ev, const uint64_t* zsum, int cha
 }
 
 hipError_t launch_split_geoms(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
-                              int* counts, hipStream_t st) {
+                              int* counts, int margin_div, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(split_geoms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, src, n, Z, stall, lists, cap,
-                     counts);
+                     counts, margin_div);
   return hipGetLastError();
 }
 
@@ -713,10 +819,10 @@ hipError_t launch_iota(int* p, int start, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_flow_reduce(const float* stage, const int* stall, int chain0, int nch, int nrows, double* out,
+hipError_t launch_flow_reduce(const float* stage, int nch, int nrows, double* out,
                               hipStream_t st) {
   if (nrows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(flow_reduce_kernel, dim3(nrows), dim3(256), 0, st, stage, stall, chain0, nch, nrows, out);
+  hipLaunchKernelGGL(flow_reduce_kernel, dim3(nrows), dim3(256), 0, st, stage, nch, nrows, out);
   return hipGetLastError();
 }
 
@@ -731,11 +837,27 @@ hipError_t launch_corr_update(const float* stage, double* D, double* A, double* 
   return hipGetLastError();
 }
 
-hipError_t launch_corr_reduce(const double* C, const int* stall, int chain0, int nch, int npts, double* out_sum,
-                              int* out_healthy, hipStream_t st) {
+hipError_t launch_corr_reduce(const double* C, int ld, const int* stall, int chain0, int first, int count, int npts,
+                              double* out_sum, int* out_healthy, hipStream_t st) {
   if (npts <= 0) return hipSuccess;
-  hipLaunchKernelGGL(corr_reduce_kernel, dim3(npts), dim3(256), 0, st, C, stall, chain0, nch, npts, out_sum,
-                     out_healthy);
+  hipLaunchKernelGGL(corr_reduce_kernel, dim3(npts), dim3(256), 0, st, C, ld, stall, chain0, first, count, npts,
+                     out_sum, out_healthy);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain_stress(const ChainArrays& ch, int chain0, int n, const float kappa[9], double* part,
+                               hipStream_t st) {
+  KappaArg K;
+  for (int i = 0; i < 9; ++i) K.k[i] = kappa[i];
+  hipLaunchKernelGGL(chain_stress_kernel, dim3(kStressBlocks), dim3(256), 0, st, ch, chain0, n, K, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_state(const ChainArrays& ch, int chain0, int n, double t_min, unsigned long long* out,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int grid = std::min(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(check_state_kernel, dim3(grid), dim3(256), 0, st, ch, chain0, n, t_min, out);
   return hipGetLastError();
 }
 

@@ -1170,7 +1170,7 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
       H.dt_prev = 0.0f;
     }
     record_samples<CPW, G, LT, EXT>(C, T, E, H, a.stage, a.ch.R1, L);
-    bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0 && !(PLAIN && H.done >= 1);
+    bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0 && !(PLAIN && H.done >= a.plain_jumps);
     if (act && H.Z >= CAP) { H.ovf = 1; act = false; }
     H.inert = H.inert || (H.ovf != 0);
     if (!__ballot(act)) break;
@@ -1332,13 +1332,15 @@ template <int CPW, int G>
 __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G>::STG; }
 
 // Persistent kernel: each wave takes work units from the queue until it is empty.
-// PLAIN: one jump per chain (the per-launch form); otherwise run to the targets.
+// PLAIN: a.plain_jumps jumps per chain, the state in HBM between launches (the
+// per-launch form; one jump per launch is the reference's structure); otherwise run to
+// the targets.  Both forms are held to the same occupancy targets.
 // EXT: the variant with the less common features, launched when an ensemble of the pass
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
 // centre-of-mass samples), the polydisperse CD background (partner tables) and flow
 // (no incremental rate refresh: the affine step recomputes every rate).
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((PLAIN || (CPW == 1 && G != 2)) ? 1 : (CPW == 1 ? 2 : 3)))) kmc_kernel(KmcArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW == 1 && G != 2) ? 1 : (CPW == 1 ? 2 : 3)))) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<CPW, G>();

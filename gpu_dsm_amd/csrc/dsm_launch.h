@@ -38,18 +38,29 @@ hipError_t launch_z_order(const int* src, int first, int n, const int* Z, const 
 // cnt[0] = n0, cnt[1..n) = 0
 hipError_t launch_init_counts(int* cnt, int n, int n0, hipStream_t st);
 hipError_t launch_split_geoms(const int* src, int n, const int* Z, const int* stall, int* lists, int cap,
-                              int* counts, hipStream_t st);
+                              int* counts, int margin_div, hipStream_t st);
 // out[0] += sum ev[chain0..chain0+n), out[1] += sum zsum[...]
 hipError_t launch_sum_counters(const uint64_t* ev, const uint64_t* zsum, int chain0, int n, unsigned long long* out,
                                hipStream_t st);
 hipError_t launch_set_ev_stop(const uint64_t* ev, uint64_t* ev_stop, int chain0, int n, uint64_t add,
                               hipStream_t st);
-hipError_t launch_flow_reduce(const float* stage, const int* stall, int chain0, int nch, int nrows, double* out,
+hipError_t launch_flow_reduce(const float* stage, int nch, int nrows, double* out,
                               hipStream_t st);
 hipError_t launch_corr_update(const float* stage, double* D, double* A, double* C, const int* stall, int chain0,
                               int nch, int p, int m, int s, int nrows, const int* cnt, int type, hipStream_t st);
-hipError_t launch_corr_reduce(const double* C, const int* stall, int chain0, int nch, int npts, double* out_sum,
-                              int* out_healthy, hipStream_t st);
+hipError_t launch_corr_reduce(const double* C, int ld, const int* stall, int chain0, int first, int count, int npts,
+                              double* out_sum, int* out_healthy, hipStream_t st);
+// ensemble stress of the current state: kStressBlocks partial sums of 9 doubles
+// (xx yy zz xy yz zx Lpp Z count over the chains that have not stalled)
+constexpr int kStressBlocks = 256;
+struct KappaArg {
+  float k[9];
+};
+hipError_t launch_chain_stress(const ChainArrays& ch, int chain0, int n, const float kappa[9], double* part,
+                               hipStream_t st);
+// invariant violations of chains chain0..chain0+n-1 added to out[0..6] (see dsm_check_state)
+hipError_t launch_check_state(const ChainArrays& ch, int chain0, int n, double t_min, unsigned long long* out,
+                              hipStream_t st);
 hipError_t launch_copy_probe(const void* in, void* out, size_t n4, int grid, hipStream_t st);
 hipError_t launch_scan_probe(const float* in, float* out, int n, int form, hipStream_t st);
 hipError_t launch_com_probe(const float* in, float* out, int n, int geom, hipStream_t st);

@@ -118,6 +118,11 @@ extern "C" int dsm_serve_plan(const dsm_serve_config* cfg, int32_t max_requests,
 
 extern "C" int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_serve_report* report,
                          double* latency_s) {
+  return dsm_serve_ex(device_id, cfg, report, latency_s, nullptr);
+}
+
+extern "C" int dsm_serve_ex(int32_t device_id, const dsm_serve_config* cfg, dsm_serve_report* report,
+                            double* latency_s, double* summary) {
   if (!valid(cfg) || !report) return DSM_ERR_ARG;
   std::vector<Req> rq = plan(cfg);
   dsm_engine* e = nullptr;
@@ -212,17 +217,29 @@ extern "C" int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_ser
       r.t = te[q];
       if (r.t < cfg->horizon) { still.push_back(active[q]); continue; }
       // result retrieval is part of the request
+      double* sm = summary ? summary + (size_t)active[q] * DSM_SERVE_SUMMARY : nullptr;
+      if (sm) std::fill(sm, sm + DSM_SERVE_SUMMARY, 0.0);
       if (r.rate > 0.0) {
         int64_t rows = 0;
         if ((rc = dsm_flow_stress(e, r.ens, 0, 0, nullptr, &rows))) break;
         flow_rows.resize((size_t)rows * 9);
         if ((rc = dsm_flow_stress(e, r.ens, 0, rows, flow_rows.data(), &rows))) break;
+        if (sm) {
+          sm[0] = 2.0;
+          sm[1] = (double)rows;
+          if (rows > 0) std::copy(flow_rows.end() - 9, flow_rows.end(), sm + 2);
+        }
       } else {
         int32_t np = 0;
         if ((rc = dsm_correlator(e, r.ens, 0, nullptr, nullptr, &np))) break;
         corr_lag.resize(np);
         corr_g.resize(np);
         if ((rc = dsm_correlator(e, r.ens, np, corr_lag.data(), corr_g.data(), &np))) break;
+        if (sm) {
+          sm[0] = 1.0;
+          sm[1] = (double)np;
+          for (int j = 0; j < 16 && j < np; ++j) sm[2 + j] = corr_g[j];
+        }
       }
       if ((rc = dsm_remove_ensemble(e, r.ens))) break;
       used -= r.bytes;

@@ -85,6 +85,7 @@ struct KmcArgs {
   int use_ev_stop;      // stop each chain when ev reaches ch.ev_stop[c]
   int lds_floats;       // per-wave LDS table copy size (0: tables read from global memory)
   int ext;              // an ensemble of the pass samples MSD(t), has a polydisperse CD background or flows
+  int plain_jumps;      // plain form: jumps per chain per launch
 };
 
 }  // namespace dsm

@@ -23,7 +23,12 @@ EXPORTS = ["dsm_create", "dsm_destroy", "dsm_last_error", "dsm_mem_info", "dsm_a
            "dsm_remove_ensemble", "dsm_init_equilibrium", "dsm_state_cap", "dsm_get_state", "dsm_set_state", "dsm_get_r1",
            "dsm_advance", "dsm_advance_many", "dsm_advance_events", "dsm_flow_stress", "dsm_correlator",
            "dsm_last_samples", "dsm_tables", "dsm_ensemble_scalars", "dsm_probe_scan", "dsm_probe_math",
-           "dsm_probe_philox", "dsm_probe_com", "dsm_probe_copy", "dsm_main", "dsm_main_ex", "dsm_serve", "dsm_serve_plan"]
+           "dsm_probe_philox", "dsm_probe_com", "dsm_probe_copy", "dsm_main", "dsm_main_ex", "dsm_serve", "dsm_serve_plan",
+           "dsm_ensemble_info", "dsm_save", "dsm_load", "dsm_load_error", "dsm_stress", "dsm_check_state",
+           "dsm_serve_ex", "dsm_correlator_range", "dsm_set_option", "dsm_get_option"]
+OPTIONS = {"plain_jumps": 1, "small_per_cu": 2, "split_margin": 3, "lds_table_floats": 4, "concurrent": 5,
+           "blocks_per_cu": 6, "window": 7}
+CHECKS = ["z_range", "n_below_1", "n_sum", "end_q", "nonfinite", "stalled", "behind"]
 
 
 class DsmError(RuntimeError):
@@ -108,6 +113,15 @@ def load_library():
                                 P(C.c_int64)]
     L.dsm_set_state.argtypes = L.dsm_get_state.argtypes
     L.dsm_get_r1.argtypes = [C.c_void_p, C.c_int32, P(C.c_float)]
+    L.dsm_ensemble_info.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32)]
+    L.dsm_save.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_double]
+    L.dsm_load.argtypes = [C.c_void_p, C.c_int32, C.c_char_p]
+    L.dsm_load_error.restype = C.c_char_p
+    L.dsm_stress.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_int32)]
+    L.dsm_check_state.argtypes = [C.c_void_p, C.c_int32, C.c_double, P(C.c_int64)]
+    L.dsm_set_option.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.dsm_get_option.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32)]
+    L.dsm_serve_ex.argtypes = [C.c_int32, P(ServeConfig), P(ServeReport), P(C.c_double), P(C.c_double)]
     L.dsm_advance.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, P(C.c_int32), P(C.c_int32),
                               P(RunStats)]
     L.dsm_advance_many.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_double), C.c_int32, C.c_int32, P(C.c_int32),
@@ -115,6 +129,8 @@ def load_library():
     L.dsm_advance_events.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, P(RunStats)]
     L.dsm_flow_stress.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int64, P(C.c_double), P(C.c_int64)]
     L.dsm_correlator.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_int32)]
+    L.dsm_correlator_range.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_double),
+                                       P(C.c_double), P(C.c_int32)]
     L.dsm_last_samples.argtypes = [C.c_void_p, C.c_int32, P(C.c_float), C.c_int64, P(C.c_int64), P(C.c_int64)]
     L.dsm_tables.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_float), P(C.c_int32)]
     L.dsm_ensemble_scalars.argtypes = [C.c_void_p, C.c_int32, P(C.c_float)]
@@ -175,6 +191,18 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+    # ------------------------------------------------------------ options
+    def set_option(self, name, value):
+        _check(self._L.dsm_set_option(self.h, OPTIONS[name], int(value)))
+
+    def get_option(self, name):
+        v = C.c_int32()
+        _check(self._L.dsm_get_option(self.h, OPTIONS[name], C.byref(v)))
+        return v.value
+
+    def options(self):
+        return {k: self.get_option(k) for k in OPTIONS}
 
     # ------------------------------------------------------------ ensembles
     def add_ensemble(self, n_chains, nk, beta=1.0, cd=0, kappa=None, seed=1, salt=0, sample="none", delta=1.0,
@@ -251,6 +279,30 @@ class Engine:
         assert arrs[0].shape[0] == n
         _check(self._L.dsm_set_state(self.h, int(ens), int(cap), *ptrs))
 
+    def stress(self, ens):
+        """Ensemble stress of the current state: (xx yy zz xy yz zx Lpp Z averages, healthy chains)."""
+        out = np.zeros(8)
+        h = C.c_int32()
+        _check(self._L.dsm_stress(self.h, int(ens), _p(out, C.c_double), C.byref(h)))
+        return out, h.value
+
+    def check(self, ens, t_min=-1.0):
+        """Invariant violations of the ensemble's state counted on the device (dict by CHECKS)."""
+        out = np.zeros(7, np.int64)
+        _check(self._L.dsm_check_state(self.h, int(ens), float(t_min), _p(out, C.c_int64)))
+        return dict(zip(CHECKS, out.tolist()))
+
+    def save(self, ens, path, t_ref=0.0):
+        """Write the -s binary file of the ensemble (times relative to t_ref)."""
+        _check(self._L.dsm_save(self.h, int(ens), str(path).encode(), float(t_ref)))
+
+    def load(self, ens, path):
+        """Read a -s / -l binary file (reference layout) into the ensemble."""
+        rc = self._L.dsm_load(self.h, int(ens), str(path).encode())
+        if rc != 0:
+            msg = self._L.dsm_load_error().decode(errors="replace") or self._L.dsm_last_error().decode(errors="replace")
+            raise DsmError("gpu_dsm_amd error %d: %s" % (rc, msg))
+
     def advance(self, ens, t_end, mode="batched", run_flag=None):
         rs = RunStats()
         flag = None
@@ -291,6 +343,23 @@ class Engine:
         _check(self._L.dsm_correlator(self.h, int(ens), npts.value, _p(lag, C.c_double), _p(g, C.c_double),
                                       C.byref(npts)))
         return lag, g
+
+    def correlator_chains(self, ens, first=0, count=None, group=1):
+        """G(t) of consecutive chain groups: returns (lags, G[n_groups][np]), group g
+        holding chains first + g*group .. first + (g+1)*group - 1."""
+        n = self.ens[ens]["n"]
+        count = n - first if count is None else count
+        npts = C.c_int32()
+        _check(self._L.dsm_correlator(self.h, int(ens), 0, None, None, C.byref(npts)))
+        ng = count // group
+        lag = np.zeros(npts.value)
+        out = np.zeros((ng, npts.value))
+        for k in range(ng):
+            g = np.zeros(npts.value)
+            _check(self._L.dsm_correlator_range(self.h, int(ens), int(first + k * group), int(group), npts.value,
+                                                _p(lag, C.c_double), _p(g, C.c_double), C.byref(npts)))
+            out[k] = g
+        return lag, out
 
     def last_samples(self, ens):
         k0, rows = C.c_int64(), C.c_int64()
@@ -400,12 +469,18 @@ def serve_plan(**kw):
     return out
 
 
-def serve(device=0, **kw):
-    """Run a serving stream on the device; returns (report dict, per-request latency in s)."""
+def serve(device=0, summary=False, **kw):
+    """Run a serving stream on the device; returns (report dict, per-request latency in s)
+    and, with summary=True, each request's result summary (DSM_SERVE_SUMMARY = 18 doubles:
+    kind 1 G(t) / 2 flow, points or rows, then G at lags 0..15 or the last flow row)."""
     cfg, c = _serve_config(**kw)
     rep = ServeReport()
     lat = np.zeros(c["n_requests"])
-    _check(load_library().dsm_serve(device, C.byref(cfg), C.byref(rep), _p(lat, C.c_double)))
+    sm = np.zeros((c["n_requests"], 18)) if summary else None
+    _check(load_library().dsm_serve_ex(device, C.byref(cfg), C.byref(rep), _p(lat, C.c_double),
+                                       _p(sm, C.c_double) if summary else None))
+    if summary:
+        return rep.as_dict(), lat, sm
     return rep.as_dict(), lat
 
 

@@ -90,6 +90,23 @@ const char* dsm_last_error(void);
 /* free/total HBM bytes of a device (serving admission). */
 int dsm_mem_info(int32_t device_id, size_t* free_bytes, size_t* total_bytes);
 
+/* Scheduling options of an engine (no counterpart in the reference, whose block size,
+   chains per launch and sync interval are compile-time constants: common/ensemble.cu:47,
+   common/textures_surfaces.h:22-24, common/ensemble_kernel.cu:41-42).  No option changes
+   a result -- every chain makes the same jumps with the same random numbers under any
+   setting -- only how passes are laid out on the device.  The autotuner
+   (gpu_dsm_amd/autotune.py) sweeps them. */
+#define DSM_OPT_PLAIN_JUMPS 1       /* plain form: jumps per chain per launch (default 1, the reference's) */
+#define DSM_OPT_SMALL_PER_CU 2      /* passes of at most this many chains per CU run in the wave form (12) */
+#define DSM_OPT_SPLIT_MARGIN 3      /* concurrent pass: chains within capacity/value of a wave-form geometry
+                                       start in the next one (8; 0 = none) */
+#define DSM_OPT_LDS_TABLE_FLOATS 4  /* rate tables go to LDS when they fit this many floats per wave (2816) */
+#define DSM_OPT_CONCURRENT 5        /* geometries side by side: -1 by the pass's lengths, 0 never, 1 always */
+#define DSM_OPT_BLOCKS_PER_CU 6     /* cap on persistent waves per CU (0 = the occupancy limit) */
+#define DSM_OPT_WINDOW 7            /* samples per staging window of ensembles added later (256) */
+int dsm_set_option(dsm_engine* e, int32_t key, int32_t value);
+int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value);
+
 /* Allocates an ensemble's chain state and tables; replaces gpu_init/init_call_block
    (common/ensemble.cu:118-208, common/ensemble_call_block.cu:55-104) and the
    p_CD set-up (common/pcd_tau.h:32-63, common/gpu_random.cu:59-104). */
@@ -111,6 +128,30 @@ int dsm_get_state(dsm_engine* e, int32_t ens, int32_t cap, int32_t* Z, int32_t* 
 int dsm_set_state(dsm_engine* e, int32_t ens, int32_t cap, const int32_t* Z, const int32_t* N, const float* Q,
                   const float* w, const double* t, const uint64_t* ev, const int32_t* stall,
                   const float* dt_prev, const int64_t* next_sample);
+/* Size, strand capacity and sample mode (DSM_SAMPLE_*) of an ensemble; any pointer may be NULL. */
+int dsm_ensemble_info(dsm_engine* e, int32_t ens, int32_t* n_chains, int32_t* cap, int32_t* sample_mode);
+/* The -s / -l binary files (int32 N_cha; per chain a 16-byte head {int32 Z, float time,
+   float unused, int32 stall}, Z float4 {Qx Qy Qz N}, Z float CD rates): dsm_save writes the
+   ensemble's state with times relative to t_ref; dsm_load reads a file in the reference's
+   layout into the ensemble (jump counters restart at 0, sampling restarts at the loaded
+   clocks) and returns DSM_ERR_MISMATCH for another chain count or DSM_ERR_IO for a
+   missing, truncated or invalid file (dsm_load_error() says which).  Replace save_to_file
+   / load_from_file (common/ensemble.cu:342-354,486-504, common/chain.cu:236-250), which
+   exit(2) on a mismatch. */
+int dsm_save(dsm_engine* e, int32_t ens, const char* path, double t_ref);
+int dsm_load(dsm_engine* e, int32_t ens, const char* path);
+const char* dsm_load_error(void);
+/* Ensemble stress of the current state: out[0..7] = averages over the chains that have
+   not stalled of xx yy zz xy yz zx Lpp Z (per chain sigma_ab = -sum 3 Q_a Q_b / N, FP64
+   from the stored FP32 state, the deformation pending from the last jump applied);
+   *healthy (may be NULL) = their count.  Replaces calc_stress (common/ensemble.cu:210-221)
+   over stress_calc (common/ensemble_kernel.cu:504-538). */
+int dsm_stress(dsm_engine* e, int32_t ens, double* out, int32_t* healthy);
+/* Integrity scan of an ensemble's state on the device (no counterpart in the reference):
+   out[0..6] = chains with Z outside 1..Nk, with a strand N < 1, with sum N != Nk, with a
+   chain-end strand vector != 0, with a non-finite strand vector or CD rate, stalled
+   chains, and chains whose clock is below t_min. */
+int dsm_check_state(dsm_engine* e, int32_t ens, double t_min, int64_t* out);
 /* MSD mode: R1[n*3], each chain's first entanglement in the fixed frame, the chain_head
    R1 field (common/chain.h:41) that the reference keeps on the device (d_R1,
    common/ensemble_call_block.cu:75,406,422).  Zero after dsm_init_equilibrium. */
@@ -118,7 +159,9 @@ int dsm_get_r1(dsm_engine* e, int32_t ens, float* r1);
 
 /* Advances every chain of the ensemble until its clock reaches t_end, recording
    samples and updating the correlator / flow stress on the way.  run_flag (may be
-   NULL) is polled between launches; progress (may be NULL) receives 0..100.
+   NULL) is polled after each sampling window (its samples consumed): when cleared the
+   call returns DSM_ABORTED with every chain at that window's end, a state a later call
+   continues exactly; progress (may be NULL) receives 0..100.
    Replaces gpu_time_step/time_step_call_block (common/ensemble.cu:223-229,
    common/ensemble_call_block.cu:106-222) and the G(t) loop of gpu_Gt_PCS /
    EQ_time_step_call_block (common/ensemble.cu:289-339,
@@ -142,6 +185,10 @@ int dsm_flow_stress(dsm_engine* e, int32_t ens, int64_t first_row, int64_t max_r
    correlator::calc and the accumulation in gpu_Gt_PCS (common/correlator.cu:146-187,
    common/ensemble.cu:309-321). */
 int dsm_correlator(dsm_engine* e, int32_t ens, int32_t max_pts, double* lag, double* g, int32_t* npts);
+/* The same over chains first .. first+count-1 of the ensemble only (the chains' own
+   mean; error bars over chain groups). */
+int dsm_correlator_range(dsm_engine* e, int32_t ens, int32_t first, int32_t count, int32_t max_pts, double* lag,
+                         double* g, int32_t* npts);
 /* The samples of the last staging window: out[(k-k0)*n*3 + chain*3 + comp]. */
 int dsm_last_samples(dsm_engine* e, int32_t ens, float* out, int64_t max_floats, int64_t* k0, int64_t* nrows);
 
@@ -228,6 +275,13 @@ typedef struct {
 
 /* Runs the stream on one device; latency_s (optional) receives n_requests values. */
 int dsm_serve(int32_t device_id, const dsm_serve_config* cfg, dsm_serve_report* report, double* latency_s);
+/* As dsm_serve; summary (optional) receives DSM_SERVE_SUMMARY doubles per request, read
+   with its result: [0] = 1 (G(t)) or 2 (flow), [1] = points or rows of the result; G(t):
+   [2..17] = G at lags 0..15 samples; flow: [2..10] = the last output row (sums over the
+   healthy chains of xx yy zz xy yz zx Lpp Z, then their count). */
+#define DSM_SERVE_SUMMARY 18
+int dsm_serve_ex(int32_t device_id, const dsm_serve_config* cfg, dsm_serve_report* report, double* latency_s,
+                 double* summary);
 /* The request stream alone (host only): arrival times, Nk, beta, chains, shear rate
    (0 = equilibrium) and the admission memory estimate of each request. */
 int dsm_serve_plan(const dsm_serve_config* cfg, int32_t max_requests, double* arrival_s, int32_t* nk, float* beta,

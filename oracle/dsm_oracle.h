@@ -198,6 +198,17 @@ int orc_corr_levels(double sim_time, double delta, int p, int m);
 int64_t orc_bench(const orc_params* p, const uint32_t key[2], int n_chains, uint32_t chain0,
                   double t_end, int n_threads, int fp64, double* seconds);
 
+/* G(t) of single chains over a long horizon without storing the samples: chains chain0 ..
+   chain0+n-1 of an equilibrium ensemble (per-chain Nk when nk_per_chain != NULL), each run
+   from its equilibrium draw to nsamples*delta in windows of the same samples the device
+   uses (a window ends at the first jump past its last sample time), every chain feeding
+   its own multi-tau correlator (p, m, s levels).  out[c*np + j] = chain c's G at point j
+   (np = p + (s-1)(p - p/m)); the ensemble G is the mean over chains.  fp64 selects the
+   FP64 chain.  Returns the jumps made; *seconds = wall time. */
+int64_t orc_gt_chains(const orc_params* p, const uint32_t key[2], int n, uint32_t chain0, const int* nk_per_chain,
+                      double delta, int64_t nsamples, int64_t window, int pm, int mm, int s, int n_threads, int fp64,
+                      double* out, double* seconds);
+
 /* ---------------- ensemble API (drives many chains; used through ctypes) ---------------- */
 typedef struct orc_ens orc_ens;
 orc_params* orc_params_new(float beta, int nk, int nmax, int cd, const float* kappa, int exp_wait);

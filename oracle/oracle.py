@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py, always as the checker.  The product package
 (gpu_dsm_amd) never imports this module.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -82,6 +83,9 @@ def lib():
         L.orc_corr_levels.restype = C.c_int
         L.orc_bench.argtypes = [C.c_void_p, C.POINTER(u32), C.c_int, u32, C.c_double, C.c_int, C.c_int, f64p]
         L.orc_bench.restype = C.c_int64
+        L.orc_gt_chains.argtypes = [C.c_void_p, C.POINTER(u32), C.c_int, u32, C.POINTER(C.c_int), C.c_double,
+                                    C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
+        L.orc_gt_chains.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -296,6 +300,32 @@ def bench(params, seed, n_chains, t_end, threads, fp64=False, chain0=0, salt=0):
     ev = lib().orc_bench(params.h, key, int(n_chains), int(chain0), float(t_end), int(threads), int(fp64),
                          C.byref(sec))
     return int(ev), sec.value
+
+
+def gt_chains(params, seed, n, T, delta=1.0, p=16, m=2, s=None, threads=None, fp64=True, chain0=0, salt=0,
+              nk_per_chain=None, window=256):
+    """Per-chain G(t) over a long horizon (no stored samples): returns (lags, G[n][np],
+    jumps, seconds).  The ensemble G is G.mean(axis=0); chains are independent, so
+    error bars follow from the spread over chains."""
+    nsamples = int(math.ceil(T / delta - 1e-12))
+    if s is None:
+        s = corr_levels(T, delta, p, m)
+    np_ = p + (s - 1) * (p - p // m)
+    out = np.zeros((n, np_))
+    key = (C.c_uint32 * 2)(seed & 0xFFFFFFFF, salt & 0xFFFFFFFF)
+    sec = C.c_double()
+    nkp = None
+    if nk_per_chain is not None:
+        nkarr = np.ascontiguousarray(nk_per_chain, np.int32)
+        nkp = _p(nkarr, C.c_int)
+    threads = threads or min(os.cpu_count() or 1, 16)
+    ev = lib().orc_gt_chains(params.h, key, int(n), int(chain0), nkp, float(delta), nsamples, int(window), int(p),
+                             int(m), int(s), int(threads), int(bool(fp64)), _p(out, C.c_double), C.byref(sec))
+    lags = []
+    for k in range(s):
+        for j in range(0 if k == 0 else p // m, p):
+            lags.append(j * m ** k)
+    return np.array(lags, np.float64), out, int(ev), sec.value
 
 
 class Correlator:

@@ -1,0 +1,186 @@
+"""The benchmark configurations on the device: bit-exact subsets against the mirror oracle
+and full-size runs checked by the model's invariants.
+
+  C3  polydisperse melt (log-normal Nk, Mn 200, PDI 1.5, clipped to [10, 2000]), melt CD
+      partners, G(t): a 2048-chain subset of the same lengths, including the longest
+      chains (Nk up to 2000, the Z <= 2048 geometry), runs through the concurrent
+      geometry pass; the full 262,144 chains keep every invariant.
+  C4  16 ensembles, Nk 50, shear rates 10^(-5 + 4e/15) per tau_K, one batched pass per
+      window: 16 x 256 chains against 16 oracle runs; the full 16 x 16,384 keep every
+      invariant and every chain reports in every row.
+  C5  serving: requests drawn like the stream (Nk 16..512, beta {0.5, 1, 2}, 1024..2048
+      chains, half sheared) served together, each request's result read back by the
+      server and compared with its own oracle run; the full 2000-request stream completes.
+Reference: common/eq_ensemble_kernel.cu:242-663 (events), common/gpu_random.cu:212-228
+(partner lengths), common/ensemble_kernel.cu:504-538 (stress).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import _bits, assert_same_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _no_violations(chk, what):
+    bad = {k: v for k, v in chk.items() if v}
+    assert not bad, "%s: invariant violations %s" % (what, bad)
+
+
+def _c3_subset(n=2048, n_long=4):
+    from gpu_dsm_amd.workloads import c3_lengths
+    full = c3_lengths(262144, seed=1)
+    longest = np.argsort(full)[-n_long:]
+    nk = np.concatenate([full[:n - n_long], full[longest]]).astype(np.int32)
+    return nk
+
+
+def test_c3_subset_parity(engine, oracle):
+    nk = _c3_subset()
+    n, nmax, T, p, m = nk.size, int(nk.max()), 4.0, 16, 2
+    assert nmax > 1024
+    s = max(1, oracle.corr_levels(T, 1.0, p, m))
+    P = oracle.Params(1.0, nmax, 1, nmax=nmax, partner="melt", nk_chains=nk)
+    ens = engine.add_ensemble(n, nmax, 1.0, 1, seed=1, sample="gt", delta=1.0, corr_p=p, corr_m=m, corr_levels=s,
+                              window=16, nk_per_chain=nk, partner="melt")
+    for name in ("PF", "PD", "PF_NK", "PD_NK", "CC"):
+        assert np.array_equal(_bits(engine.table(ens, name)), _bits(P.table(name))), name
+    engine.init_equilibrium(ens)
+    O = oracle.Ensemble(P, seed=1, n=n, nk_per_chain=nk)
+    assert_same_state(engine.state(ens), O.state(), "C3 subset initial")
+    _, samples = O.advance(t_end=T, delta=1.0, k0=0, k_end=int(T), samples=True)
+    st = engine.advance(ens, T)
+    g = engine.state(ens)
+    assert_same_state(g, O.state(), "C3 subset at T")
+    assert g["Z"].max() > 896, "the longest chains should need the Z <= 2048 geometry"
+    assert ((g["Z"] > 64) & (g["Z"] <= 128)).any() and ((g["Z"] > 128) & (g["Z"] <= 256)).any()
+    assert st["events"] == int(O.state()["ev"].sum())
+    k0, last = engine.last_samples(ens)
+    assert np.array_equal(_bits(last), _bits(samples[k0:k0 + last.shape[0]]))
+    corr = oracle.Correlator(p, m, s, n)
+    for row in samples:
+        corr.add_row(row)
+    lag_o, g_o = corr.result()
+    lag, gt = engine.correlator(ens)
+    assert np.array_equal(lag, lag_o)
+    np.testing.assert_allclose(gt, g_o, rtol=1e-10, atol=1e-12)
+    engine.remove_ensemble(ens)
+
+
+def _c4_rates(k=16):
+    return [10.0 ** (-5.0 + 4.0 * q / (k - 1)) for q in range(k)]
+
+
+def test_c4_subset_parity(engine, oracle):
+    n, nk, T, dlt = 256, 50, 10.0, 1.0
+    ids, refs = [], []
+    for q, gd in enumerate(_c4_rates()):
+        kappa = [0, gd, 0, 0, 0, 0, 0, 0, 0]
+        ens = engine.add_ensemble(n, nk, 1.0, 1, kappa=kappa, seed=1, salt=q, sample="flow", delta=dlt, window=16)
+        engine.init_equilibrium(ens)
+        O = oracle.Ensemble(oracle.Params(1.0, nk, 1, kappa=kappa), seed=1, n=n, salt=q)
+        _, smp = O.advance(t_end=T, delta=dlt, k0=1, k_end=int(T / dlt) + 1, samples=True, flow_samples=True)
+        ids.append(ens)
+        refs.append((O, smp))
+    engine.advance_many(ids, T)
+    for q, (ens, (O, smp)) in enumerate(zip(ids, refs)):
+        assert_same_state(engine.state(ens), O.state(), "C4 ensemble %d" % q)
+        rows = engine.flow_stress(ens)
+        assert rows.shape[0] == int(T / dlt)
+        np.testing.assert_allclose(rows[:, :8], smp.astype(np.float64).sum(axis=1), rtol=1e-12, atol=1e-9)
+        assert np.all(rows[:, 8] == n)
+    for ens in ids:
+        engine.remove_ensemble(ens)
+
+
+SERVE_MIX = dict(n_requests=10, arrival_rate=1000.0, seed=5, nk_min=16, nk_max=512, nk_step=16, beta=(0.5, 1.0, 2.0),
+                 nch_min=1024, nch_max=2048, flow_fraction=0.5, rate_min=1e-5, rate_max=1e-2, horizon=5.0,
+                 window=5.0, cd=1, hbm_fraction=0.5)
+
+
+def test_c5_served_requests_match_oracle(oracle):
+    """Requests served together (continuous batching over one engine), each result as
+    the server read it back (G(t) at lags 0..15, or the last flow output row) against
+    the request's own oracle run: seed = the stream's seed, salt = request index."""
+    import gpu_dsm_amd as g
+    rep, lat, sm = g.serve(summary=True, **SERVE_MIX)
+    plan = g.serve_plan(**SERVE_MIX)
+    nreq = SERVE_MIX["n_requests"]
+    assert rep["completed"] == nreq
+    assert rep["max_active"] >= 2, "requests should overlap on the device"
+    flow = plan["shear_rate"] > 0
+    assert flow.any() and (~flow).any()
+    T = SERVE_MIX["horizon"]
+    for i in range(nreq):
+        nk, beta, n, rate = int(plan["nk"][i]), float(plan["beta"][i]), int(plan["n_chains"][i]), plan["shear_rate"][i]
+        kappa = [0, rate, 0, 0, 0, 0, 0, 0, 0]
+        P = oracle.Params(beta, nk, 1, kappa=kappa)
+        O = oracle.Ensemble(P, seed=SERVE_MIX["seed"], n=n, salt=i)
+        if rate > 0:
+            _, smp = O.advance(t_end=T, delta=1.0, k0=1, k_end=int(T) + 1, samples=True, flow_samples=True)
+            assert sm[i, 0] == 2 and sm[i, 1] == int(T)
+            np.testing.assert_allclose(sm[i, 2:10], smp[-1].astype(np.float64).sum(axis=0), rtol=1e-12, atol=1e-9)
+            assert sm[i, 10] == n
+        else:
+            _, smp = O.advance(t_end=T, delta=1.0, k0=0, k_end=int(T), samples=True)
+            corr = oracle.Correlator(16, 2, 1, n)
+            for row in smp:
+                corr.add_row(row)
+            _, g_o = corr.result()
+            assert sm[i, 0] == 1 and sm[i, 1] == g_o.size
+            np.testing.assert_allclose(sm[i, 2:18], g_o[:16], rtol=1e-10, atol=1e-12)
+
+
+# ------------------------------------------------------------------ full sizes
+def test_c3_full_size_invariants(engine):
+    from gpu_dsm_amd.workloads import c3_lengths
+    nk = c3_lengths(262144, seed=1)
+    T = 8.0
+    ens = engine.add_ensemble(nk.size, int(nk.max()), 1.0, 1, seed=1, sample="gt", delta=1.0, corr_p=16, corr_m=2,
+                              corr_levels=1, window=8, nk_per_chain=nk, partner="melt")
+    engine.init_equilibrium(ens)
+    _no_violations(engine.check(ens), "C3 initial")
+    st = engine.advance(ens, T)
+    _no_violations(engine.check(ens, t_min=T), "C3 at T")
+    assert st["events"] > 0
+    lag, gt = engine.correlator(ens)
+    assert np.all(np.isfinite(gt)) and gt[0] > 0
+    engine.remove_ensemble(ens)
+
+
+def test_c4_full_size_invariants(engine):
+    n, nk, T = 16384, 50, 10.0
+    ids = []
+    for q, gd in enumerate(_c4_rates()):
+        ens = engine.add_ensemble(n, nk, 1.0, 1, kappa=[0, gd, 0, 0, 0, 0, 0, 0, 0], seed=1, salt=q, sample="flow",
+                                  delta=1.0, window=16)
+        engine.init_equilibrium(ens)
+        ids.append(ens)
+    engine.advance_many(ids, T)
+    for q, ens in enumerate(ids):
+        _no_violations(engine.check(ens, t_min=T), "C4 ensemble %d" % q)
+        rows = engine.flow_stress(ens)
+        assert rows.shape[0] == int(T) and np.all(rows[:, 8] == n)
+    # stress grows with the shear rate (reference sign: sigma_xy < 0 under positive shear)
+    sxy = [engine.flow_stress(ens)[-1, 3] / n for ens in ids]
+    assert sxy[-1] < sxy[0] - 0.05
+    for ens in ids:
+        engine.remove_ensemble(ens)
+
+
+def test_c5_full_stream_completes():
+    """All 2000 requests of the stream (Nk 16..512, 1024..65536 chains, half sheared),
+    arrivals compressed to 200/s and a 1 tau_K horizon, complete; latencies are ordered."""
+    import gpu_dsm_amd as g
+    kw = dict(arrival_rate=200.0, horizon=1.0, window=1.0, hbm_fraction=0.8)
+    rep, lat = g.serve(**kw)
+    plan = g.serve_plan(**kw)
+    assert rep["completed"] == 2000 == plan["nk"].size
+    assert np.all(lat > 0) and np.all(np.isfinite(lat))
+    assert rep["p50_s"] <= rep["p99_s"] <= rep["max_s"] + 1e-12
+    assert rep["jumps"] > 0 and rep["max_active"] >= 2
+    assert rep["makespan_s"] >= plan["arrival_s"][-1]
+    print("C5 full stream (compressed): p50 %.3f s p99 %.3f s max active %d, %.3g jumps/s" % (
+        rep["p50_s"], rep["p99_s"], rep["max_active"], rep["jumps_per_s"]))

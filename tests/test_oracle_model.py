@@ -308,3 +308,26 @@ def test_flow_samples_stay_inside_the_window(oracle):
     oracle.lib().orc_ens_advance(E.h, 12.0, 2 ** 62, 3.0, k0, k_end, inner.ctypes.data_as(C.c_void_p), 2)
     assert np.all(buf[0] == -7.0) and np.all(buf[-1] == -7.0)
     assert np.all(inner[:, :, 7] >= 1)          # every row written (Z column)
+
+
+def test_long_horizon_gt_runner_matches_ensemble_path(oracle):
+    """orc_gt_chains (one chain at a time, windows of samples fed to a per-chain C
+    correlator, nothing stored) gives the same G(t) as the ensemble path's samples fed
+    to the brute-force-pinned numpy correlator, for any window length."""
+    P = oracle.Params(1.0, 25, 0)
+    n, T = 64, 300.0
+    s = oracle.corr_levels(T, 1.0, 16, 2)
+    E = oracle.Ensemble(P, seed=1, n=n)
+    _, smp = E.advance(t_end=T, delta=1.0, k0=0, k_end=int(T), samples=True)
+    cr = oracle.Correlator(16, 2, s, n)
+    for row in smp:
+        cr.add_row(row)
+    lag_o, g_o = cr.result()
+    for window, fp64 in ((37, False), (256, False), (64, True)):
+        lags, G, ev, _ = oracle.gt_chains(P, 1, n, T, s=s, fp64=fp64, window=window)
+        assert np.array_equal(lags, lag_o)
+        if not fp64:
+            assert ev == int(E.state()["ev"].sum())
+            np.testing.assert_allclose(G.mean(axis=0), g_o, rtol=1e-12, atol=1e-12)
+        else:
+            assert abs(G.mean(axis=0)[0] - g_o[0]) < 0.01 * g_o[0]
