@@ -1,0 +1,211 @@
+"""Autotuner for the engine's scheduling options (dsm_set_option).
+
+The reference fixes its launch shape at compile time (threads per block and chains per
+call, common/textures_surfaces.h:22-24, common/ensemble.cu:47; sync interval,
+common/ensemble_kernel.cu:41-42).  Here the per-pass layout is chosen per workload:
+
+  small_per_cu      passes of few chains run in the wave form (lowest per-jump latency)
+                    instead of the row/pair forms (most jumps per instruction);
+  concurrent        geometries side by side on four streams or one after another;
+  split_margin      how close to a wave-form geometry's capacity a chain may start in it;
+  lds_table_floats  rate tables staged in LDS (one copy per wave) or read from global memory;
+  blocks_per_cu     persistent waves per CU (occupancy below the register limit);
+  window            samples per staging window: the stress / correlator sync interval.
+
+No option changes a result (every chain makes the same jumps with the same Philox
+numbers under any setting; tests/test_gpu_options.py), so tuning is free to pick on
+speed alone.  tune() builds the workload in a scratch engine from the same ensemble
+descriptions, runs one warm-up window, then for each option in turn times a window per
+candidate value (all others held at their best so far: coordinate descent) and keeps
+the fastest.  Results are cached in a JSON file keyed by the workload's signature
+(form, flow, partners, beta, log2 of the mean Nk and of the chain count).
+
+    python -m gpu_dsm_amd.autotune c1|c2|c3|c4 [--cache FILE]   one JSON line: default vs tuned
+"""
+import argparse
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import engine as eng
+
+CANDIDATES = {
+    "small_per_cu": [12, 0, 4, 24, 48],
+    "concurrent": [-1, 0, 1],
+    "split_margin": [8, 0, 4, 16],
+    "lds_table_floats": [2816, 0, 8192],
+    "blocks_per_cu": [0, 1, 2],
+    "window": [256, 16, 64],
+}
+DEFAULTS = {"small_per_cu": 12, "concurrent": -1, "split_margin": 8, "lds_table_floats": 2816, "blocks_per_cu": 0,
+            "window": 256}
+
+
+def signature(descs):
+    """Cache key of a workload: the features the layout depends on."""
+    n = sum(d["n_chains"] for d in descs)
+    nks = np.concatenate([np.asarray(d.get("nk_per_chain"), np.float64) if d.get("nk_per_chain") is not None
+                          else np.full(d["n_chains"], float(d["nk"])) for d in descs])
+    flow = any(d.get("kappa") is not None and np.any(np.asarray(d["kappa"]) != 0) for d in descs)
+    partner = sorted({str(d.get("partner")) for d in descs})
+    betas = sorted({float(d.get("beta", 1.0)) for d in descs})
+    return "ens%d:flow%d:%s:beta%s:lognk%.1f:logn%d:poly%d" % (
+        min(len(descs), 16), int(flow), "+".join(partner), "+".join("%g" % b for b in betas),
+        round(2 * math.log2(nks.mean())) / 2, round(math.log2(n)), int(nks.std() > 0.1 * nks.mean()))
+
+
+def _load(path):
+    try:
+        return json.load(open(path))
+    except Exception:
+        return {}
+
+
+def _store(path, key, entry):
+    d = _load(path)
+    d[key] = entry
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    tmp = path + ".tmp"
+    json.dump(d, open(tmp, "w"), indent=1)
+    os.replace(tmp, path)
+
+
+def default_cache():
+    return os.environ.get("DSM_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "gpu_dsm_amd",
+                                                         "tune.json"))
+
+
+def _build(e, descs, window):
+    ids = []
+    for d in descs:
+        kw = dict(d)
+        kw.setdefault("window", window)
+        ids.append(e.add_ensemble(**kw))
+        e.init_equilibrium(ids[-1])
+    return ids
+
+
+def _rate(e, ids, t, dt, reps):
+    """jumps/s over `reps` steps of dt tau_K (wall clock, so every per-window cost counts)."""
+    ev, wall = 0, 0.0
+    for _ in range(reps):
+        t += dt
+        t0 = time.perf_counter()
+        st = e.advance_many(ids, t)
+        wall += time.perf_counter() - t0
+        ev += st["events"]
+    return ev / wall, t
+
+
+def tune(descs, step_tau=8.0, reps=3, device=0, cache=None, use_cache=True, knobs=None, min_gain=0.03):
+    """Options for the workload `descs` (a list of Engine.add_ensemble keyword dicts).
+    Returns dict(key, best, default_rate, best_rate, noise, trials); applies nothing.
+
+    The default setting is timed at the start and again at the end of the sweep; a
+    candidate replaces the best so far only when it is faster by more than `min_gain` and
+    by more than the spread of those default timings (run-to-run noise, mostly the
+    latency-bound long chains of a window)."""
+    cache = cache or default_cache()
+    key = signature(descs)
+    if use_cache:
+        hit = _load(cache).get(key)
+        if hit:
+            return dict(hit, key=key, cached=True)
+    knobs = knobs or list(CANDIDATES)
+    best = dict(DEFAULTS)
+    trials = []
+    with eng.Engine(device) as e:
+        for k, v in best.items():
+            e.set_option(k, v)
+        ids = _build(e, descs, best["window"])
+        t = 0.0
+        _, t = _rate(e, ids, t, step_tau, 1)  # warm-up
+        base, t = _rate(e, ids, t, step_tau, reps)
+        trials.append(dict(best, rate=base))
+        best_rate = base
+        gain = 1.0 + min_gain
+        for knob in knobs:
+            if knob == "window":
+                continue  # fixed when an ensemble is added: tried below in fresh engines
+            for v in CANDIDATES[knob]:
+                if v == best[knob]:
+                    continue
+                e.set_option(knob, v)
+                r, t = _rate(e, ids, t, step_tau, reps)
+                trials.append(dict(best, **{knob: v}, rate=r))
+                if r > best_rate * gain:
+                    best_rate, best[knob] = r, v
+            e.set_option(knob, best[knob])
+        for k, v in DEFAULTS.items():
+            e.set_option(k, v)
+        base2, t = _rate(e, ids, t, step_tau, reps)
+        trials.append(dict(DEFAULTS, rate=base2, note="default, timed again after the sweep"))
+    noise = abs(base2 - base) / min(base, base2)
+    if best != DEFAULTS and best_rate < max(base, base2) * (1.0 + max(min_gain, noise)):
+        best, best_rate = dict(DEFAULTS), max(base, base2)  # within the noise: keep the defaults
+    gain = 1.0 + max(min_gain, noise)
+    if "window" in knobs:
+        for v in CANDIDATES["window"]:
+            if v == best["window"]:
+                continue
+            with eng.Engine(device) as e:
+                for k, val in best.items():
+                    e.set_option(k, val)
+                e.set_option("window", v)
+                ids = _build(e, descs, v)
+                _, t = _rate(e, ids, 0.0, step_tau, 1)
+                r, t = _rate(e, ids, t, step_tau, reps)
+            trials.append(dict(best, window=v, rate=r))
+            if r > best_rate * gain:
+                best_rate, best["window"] = r, v
+    entry = dict(best=best, default_rate=max(base, base2), best_rate=best_rate, noise=noise, trials=trials,
+                 step_tau=step_tau)
+    _store(cache, key, entry)
+    return dict(entry, key=key, cached=False)
+
+
+def apply(e, options):
+    """Set tuned options on an engine (before its ensembles are added, for `window`)."""
+    for k, v in options.items():
+        e.set_option(k, v)
+
+
+# ------------------------------------------------------------------ workloads
+def workload(name, seed=1):
+    from .workloads import c3_lengths
+    if name == "c1":
+        return [dict(n_chains=1000, nk=25, beta=1.0, cd=0, seed=seed, sample="gt", delta=1.0, corr_p=16, corr_m=2,
+                     corr_levels=4)]
+    if name == "c2":
+        return [dict(n_chains=131072, nk=100, beta=1.0, cd=1, seed=seed, sample="gt", delta=1.0, corr_p=16, corr_m=2,
+                     corr_levels=4)]
+    if name == "c3":
+        nk = c3_lengths(262144, seed=seed)
+        return [dict(n_chains=nk.size, nk=int(nk.max()), beta=1.0, cd=1, seed=seed, sample="gt", delta=1.0, corr_p=16,
+                     corr_m=2, corr_levels=4, nk_per_chain=nk, partner="melt")]
+    if name == "c4":
+        rates = [10.0 ** (-5.0 + 4.0 * q / 15) for q in range(16)]
+        return [dict(n_chains=16384, nk=50, beta=1.0, cd=1, kappa=[0, r, 0, 0, 0, 0, 0, 0, 0], seed=seed, salt=q,
+                     sample="flow", delta=1.0) for q, r in enumerate(rates)]
+    raise ValueError(name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--cache", default=None)
+    ap.add_argument("--step", type=float, default=None, help="tau_K per timed step")
+    ap.add_argument("--fresh", action="store_true", help="ignore a cached result")
+    args = ap.parse_args()
+    step = args.step or {"c1": 64.0, "c2": 16.0, "c3": 8.0, "c4": 16.0}[args.which]
+    res = tune(workload(args.which), step_tau=step, cache=args.cache, use_cache=not args.fresh)
+    res["workload"] = args.which
+    res["speedup"] = res["best_rate"] / res["default_rate"]
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()
