@@ -25,7 +25,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["dsm_kernels.hip", "dsm_engine.cpp", "dsm_driver.cpp", "dsm_serve.cpp"]
+SOURCES = ["dsm_kernels.hip", "dsm_engine.cpp", "dsm_driver.cpp", "dsm_serve.cpp", "dsm_noalloc.hip"]
 GEOMS = [0, 1, 2, 3, 4, 5]  # KMC chain geometries (one translation unit each)
 
 

@@ -25,7 +25,7 @@ EXPORTS = ["dsm_create", "dsm_destroy", "dsm_last_error", "dsm_mem_info", "dsm_a
            "dsm_last_samples", "dsm_tables", "dsm_ensemble_scalars", "dsm_probe_scan", "dsm_probe_math",
            "dsm_probe_philox", "dsm_probe_com", "dsm_probe_copy", "dsm_main", "dsm_main_ex", "dsm_serve", "dsm_serve_plan",
            "dsm_ensemble_info", "dsm_save", "dsm_load", "dsm_load_error", "dsm_stress", "dsm_check_state",
-           "dsm_serve_ex", "dsm_correlator_range", "dsm_set_option", "dsm_get_option"]
+           "dsm_serve_ex", "dsm_correlator_range", "dsm_set_option", "dsm_get_option", "dsm_stress_noalloc"]
 OPTIONS = {"plain_jumps": 1, "small_per_cu": 2, "split_margin": 3, "lds_table_floats": 4, "concurrent": 5,
            "blocks_per_cu": 6, "window": 7}
 CHECKS = ["z_range", "n_below_1", "n_sum", "end_q", "nonfinite", "stalled", "behind"]
@@ -83,6 +83,11 @@ SERVE_DEFAULTS = dict(n_requests=2000, arrival_rate=20.0, seed=1, nk_min=16, nk_
                       progress_s=0.0)
 
 
+class DeviceChains(C.Structure):
+    """dsm_device_chains: device addresses of a chain state in the caller's memory."""
+    _fields_ = [(f, C.c_void_p) for f in ("Z", "stall", "base", "dt_prev", "Qx", "Qy", "Qz", "N")]
+
+
 class MainOptions(C.Structure):
     _fields_ = [("mode", C.c_int32), ("corr_p", C.c_int32), ("corr_m", C.c_int32), ("window", C.c_int32),
                 ("exp_wait", C.c_int32), ("quiet", C.c_int32), ("workdir", C.c_char_p)]
@@ -119,6 +124,8 @@ def load_library():
     L.dsm_load_error.restype = C.c_char_p
     L.dsm_stress.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), P(C.c_int32)]
     L.dsm_check_state.argtypes = [C.c_void_p, C.c_int32, C.c_double, P(C.c_int64)]
+    L.dsm_stress_noalloc.argtypes = [C.c_int32, P(DeviceChains), C.c_int32, P(C.c_float), P(C.c_double),
+                                     P(C.c_int32)]
     L.dsm_set_option.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     L.dsm_get_option.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32)]
     L.dsm_serve_ex.argtypes = [C.c_int32, P(ServeConfig), P(ServeReport), P(C.c_double), P(C.c_double)]
@@ -433,6 +440,19 @@ def probe_philox(ctr_key, device=0):
     out = np.zeros((ck.shape[0], 4), np.uint32)
     _check(load_library().dsm_probe_philox(device, _p(ck, C.c_uint32), _p(out, C.c_uint32), ck.shape[0]))
     return out
+
+
+def stress_noalloc(addresses, n_chains, kappa=None, device=0):
+    """dsm_stress_noalloc over a chain state in device memory the caller owns:
+    `addresses` maps Z, stall, base, dt_prev, Qx, Qy, Qz, N to device addresses (ints).
+    Returns (xx yy zz xy yz zx Lpp Z averages, healthy chains)."""
+    dc = DeviceChains(*[int(addresses[f]) for f, _ in DeviceChains._fields_])
+    k = np.zeros(9, np.float32) if kappa is None else np.ascontiguousarray(kappa, np.float32)
+    out = np.zeros(8)
+    h = C.c_int32()
+    _check(load_library().dsm_stress_noalloc(int(device), C.byref(dc), int(n_chains), _p(k, C.c_float),
+                                             _p(out, C.c_double), C.byref(h)))
+    return out, h.value
 
 
 def probe_copy(n_bytes, reps=20, device=0):

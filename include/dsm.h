@@ -152,6 +152,29 @@ int dsm_stress(dsm_engine* e, int32_t ens, double* out, int32_t* healthy);
    chain-end strand vector != 0, with a non-finite strand vector or CD rate, stalled
    chains, and chains whose clock is below t_min. */
 int dsm_check_state(dsm_engine* e, int32_t ens, double t_min, int64_t* out);
+/* Chain state in the caller's own device memory, structure of arrays: per chain Z,
+   stall flag, base (index of strand 0 in the strand arrays) and dt_prev (the waiting time
+   of its last jump, whose deformation is still pending); per strand slot Qx, Qy, Qz, N. */
+typedef struct {
+  const int32_t* Z;
+  const int32_t* stall;
+  const int64_t* base;
+  const float* dt_prev;
+  const float* Qx;
+  const float* Qy;
+  const float* Qz;
+  const int32_t* N;
+} dsm_device_chains;
+/* dsm_stress for chain state in the caller's device memory, allocating nothing: no
+   counterpart in the reference; for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops).  Same kernel and result as dsm_stress (out[0..7] =
+   averages over the chains that have not stalled of xx yy zz xy yz zx Lpp Z, kappa the
+   velocity gradient for the pending deformation, NULL = none).  Its partial sums live in
+   a fixed device array of 256 x 9 doubles (18 KiB) on each GPU the library runs on, kept
+   until the process ends; every chain count fits it (the grid is fixed), and calls that
+   overlap on one GPU share it and race.  Synchronous on the device's null stream. */
+int dsm_stress_noalloc(int32_t device_id, const dsm_device_chains* chains, int32_t n_chains, const float* kappa,
+                       double* out, int32_t* healthy);
 /* MSD mode: R1[n*3], each chain's first entanglement in the fixed frame, the chain_head
    R1 field (common/chain.h:41) that the reference keeps on the device (d_R1,
    common/ensemble_call_block.cu:75,406,422).  Zero after dsm_init_equilibrium. */
