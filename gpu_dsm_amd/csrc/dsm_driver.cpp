@@ -1,8 +1,6 @@
 // dsm_driver.cpp -- drop-in replacement for main_cuda (ktaletsk/gpu_dsm
 // common/main_cuda.cu:34-185): input.dat parsing, flow / G(t) dispatch, the
 // tau/G[_ID].dat, distr_*.dat and -s/-l binary formats, stdout lines.
-#include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>

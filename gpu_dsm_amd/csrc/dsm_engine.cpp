@@ -332,6 +332,7 @@ struct EngineOptions {
   int concurrent = -1;                   // DSM_OPT_CONCURRENT
   int blocks_per_cu = 0;                 // DSM_OPT_BLOCKS_PER_CU
   int window = kDefaultWindow;           // DSM_OPT_WINDOW
+  int graph = 0;                         // DSM_OPT_GRAPH
 };
 
 struct dsm_engine {
@@ -388,6 +389,21 @@ struct dsm_engine {
   double* d_part = nullptr;  // per-block partial sums of dsm_stress
   hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> host_work;
+  // Captured window passes (DSM_OPT_GRAPH).  A graph holds device pointers and the
+  // engine's options; `gen` counts their changes, and a graph is replayed only while
+  // the generation it was captured in is current.  Its window parameters come from the
+  // pinned block h_win, rewritten before every replay.
+  uint64_t gen = 1;
+  bool capturing = false;
+  struct Captured {
+    std::vector<int64_t> key;  // gen, then (ensemble, rows) per ensemble of the window
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+  };
+  std::vector<Captured> graphs;
+  uint64_t graph_clock = 0;
+  void* h_win = nullptr;
+  size_t h_win_cap = 0;
 };
 
 namespace {
@@ -427,6 +443,7 @@ constexpr size_t kStrandBytes = 5 * 4;                             // Qx Qy Qz N
 int grow_chains(dsm_engine* e, int n_need, int64_t s_need) {
   ChainArrays& c = e->ch;
   int rc = 0;
+  e->gen++;  // chain arrays or work lists may move
   if (n_need > e->nch) {
     const int n0 = e->nch;
     const int n_new = (int)std::min<int64_t>(grown_capacity(n0, n_need, kChainBytes), INT32_MAX / 2);
@@ -468,6 +485,7 @@ int grow_chains(dsm_engine* e, int n_need, int64_t s_need) {
 int upload_ens(dsm_engine* e) {
   const int n = (int)e->ens.size();
   if (n > e->d_ens_cap) {
+    e->gen++;
     if (e->d_ens) HIPCHK(hipFree(e->d_ens));
     int rc = dev_alloc(&e->d_ens, (size_t)std::max(n, 16));
     if (rc) return rc;
@@ -483,6 +501,7 @@ int upload_ens(dsm_engine* e) {
 // floats [off, off+n) of the table arena to the device (all of it when the device copy grows)
 int upload_tabs(dsm_engine* e, int64_t off, int64_t n) {
   if (e->tabs.size() > e->d_tabs_cap) {
+    e->gen++;
     if (e->d_tabs) HIPCHK(hipFree(e->d_tabs));
     e->d_tabs = nullptr;
     e->d_tabs_cap = 0;
@@ -543,6 +562,7 @@ int take_tables(dsm_engine* e, const dsm_engine::TabSlot& key, Make make, int* s
 
 int ensure_stage(dsm_engine* e) {
   if (e->stage_used > e->d_stage_cap) {  // keeps the rows of live ensembles (last_samples)
+    e->gen++;
     const int64_t cap = grown_capacity(e->d_stage_cap, e->stage_used, sizeof(float));
     int rc = grow(&e->d_stage, e->d_stage_cap, cap, e->st);
     if (rc) return rc;
@@ -559,6 +579,7 @@ Ens* get_ens(dsm_engine* e, int id) {
 // correlator memory, sized when the level count is known
 int corr_alloc(dsm_engine* e, Ens& E, int s) {
   if (E.s == s && E.dD) return DSM_OK;
+  e->gen++;
   if (E.dD) { (void)hipFree(E.dD); (void)hipFree(E.dA); (void)hipFree(E.dC); E.dD = E.dA = E.dC = nullptr; }
   E.s = s;
   const size_t n = (size_t)E.n;
@@ -725,39 +746,45 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   };
   auto grid = [&](int b) { return device_blocks(e, b, false, a.lds_floats, a.ext != 0); };
   auto late_of = [&](int b) { return kLateList + (b < kLateFirst ? kLateFirst : b + 1); };  // bin b's overflow
-  HIPCHK(hipEventRecord(e->cev[0], e->st));
+  const bool timed = !e->capturing;  // a captured pass is timed as a whole, around its replay
+  if (timed) HIPCHK(hipEventRecord(e->cev[0], e->st));
   HIPCHK(launch_fill_u32(cnt, 0u, 2 * kNumLists, e->st));
   HIPCHK(launch_split_geoms(list(0), ps.n0, e->ch.Z, e->ch.stall, list(kSplitList), (int)cap, cnt + kSplitList,
                             e->opt.split_margin_div, e->st));
   HIPCHK(hipEventRecord(e->ev_split, e->st));
-  // first launches.  A process has four hardware queues on this device, so four streams: the
-  // row form and the pair form on their own; the two long-chain geometries (few chains,
-  // latency-bound runs) start at once on the other two, each followed by a wave-form
-  // bulk geometry (G = 4 then G = 2, G = 8 then G = 1).  Measured on the polydisperse
-  // melt, this ends the window soonest of the mappings tried (269 ms; 276 and 280 ms
-  // for two others): the first round keeps the device's VALUs busy to its end, so the
-  // window is set by the total work more than by the order.
-  static const int kStreamOf[kNumBins] = {0, 3, 2, 1, 1, 2};
+  // first launches.  A process has four hardware queues on this device, so four streams:
+  // the row and pair forms in one merged launch on the engine stream; the longest chains
+  // (G = 8: few chains, a latency-bound run that sets the window's critical path) alone
+  // on one; the wave-form bulk of G = 1 on another; G = 4 (latency-bound) followed by
+  // the G = 2 bulk on the last.  Every geometry thus starts at the split.  (With the row
+  // and pair forms on separate queues, the G = 1 bulk had to wait for the G = 8 run: on
+  // the polydisperse melt a 16 tau_K window took 124-152 ms, its G = 8 kernel 67-98 ms
+  // of it.)
+  static const int kStreamOf[kNumBins] = {0, 0, 3, 1, 1, 2};
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
   auto stream = [&](int k) { return k == 0 ? e->st : e->xs[k]; };  // xs[0] is not a stream
-  for (int b = ps.bin_max; b >= 1; --b) {
+  for (int b = ps.bin_max; b >= 2; --b) {
     bind(kSplitList + b, late_of(b));
     HIPCHK(launch_kmc(a, b, false, grid(b), stream(kStreamOf[b])));
   }
   for (int k = 1; k < 4; ++k) HIPCHK(hipEventRecord(e->ev_bin[k], e->xs[k]));
+  bind(kSplitList + 1, late_of(1));
+  const KmcArgs pair_args = a;
   bind(kSplitList, late_of(0));
-  HIPCHK(hipEventRecord(e->cev[1], e->st));
-  HIPCHK(launch_kmc(a, 0, false, grid(0), e->st));
-  HIPCHK(hipEventRecord(e->cev[2], e->st));
+  int nb = kmc_rowpair_occupancy(a.lds_floats, a.ext != 0);
+  if (e->opt.blocks_per_cu > 0) nb = std::min(nb, e->opt.blocks_per_cu);
+  if (timed) HIPCHK(hipEventRecord(e->cev[1], e->st));
+  HIPCHK(launch_kmc_rowpair(a, pair_args, std::max(nb, 1) * e->n_cu, e->st));
+  if (timed) HIPCHK(hipEventRecord(e->cev[2], e->st));
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[k], 0));
   // late launches: chains that outgrew their geometry, through the larger ones in order
   for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) {
     bind(kLateList + b, late_of(b));
     HIPCHK(launch_kmc(a, b, false, grid(b), e->st));
   }
-  HIPCHK(hipEventRecord(e->cev[3], e->st));
-  ps.pending = 2;  // read by pass_times after the caller's next synchronisation
-  rs->kmc_launches += 1 + ps.bin_max + (late_last(ps.bin_max) - kLateFirst + 1);
+  if (timed) HIPCHK(hipEventRecord(e->cev[3], e->st));
+  ps.pending = timed ? 2 : 0;  // read by pass_times after the caller's next synchronisation
+  rs->kmc_launches += ps.bin_max + (late_last(ps.bin_max) - kLateFirst + 1);
   rs->first_launches++;
   return DSM_OK;
 }
@@ -867,7 +894,7 @@ int kmc_pass(dsm_engine* e, const std::vector<int>& ids, int mode, bool use_ev_s
     }
     return DSM_OK;
   }
-  if ((rc = pass_bins(e, ps, false, true, rs))) return rc;
+  if ((rc = pass_bins(e, ps, false, !e->capturing, rs))) return rc;
   if (later) *later = ps;
   else if ((rc = pass_times(e, ps, rs))) return rc;
   return DSM_OK;
@@ -881,6 +908,109 @@ int finish_stats(dsm_engine* e, const std::vector<int>& ids, const unsigned long
   rs->events = c[0] - before[0];
   rs->z_sum = c[1] - before[1];
   rs->total_ms = T.ms();
+  return DSM_OK;
+}
+
+// The device work of one window after its parameters are set: staging fills, the KMC
+// pass and the sample consumers (correlator update, flow-row reduction).  Parameters come
+// from the host (eager: uploaded) or, in a captured pass, from the pinned block h_win
+// through stage_window_kernel at the head of the graph.
+int window_launches(dsm_engine* e, const std::vector<int>& act, const std::vector<int64_t>& cnt_off,
+                    const std::vector<int64_t>& row_off, const std::vector<int>& cnt, int mode, dsm_run_stats* rs,
+                    Pass* timed) {
+  int rc;
+  if (e->capturing) {
+    const WinParam* wp = (const WinParam*)e->h_win;
+    HIPCHK(launch_stage_window(wp, (int)act.size(), e->d_ens, (const int*)(wp + act.size()), e->d_corr_cnt,
+                               (int)cnt.size(), e->st));
+  } else {
+    if ((rc = upload_ens(e))) return rc;
+    if (!cnt.empty())
+      HIPCHK(hipMemcpyAsync(e->d_corr_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, e->st));
+  }
+  for (int id : act) {
+    Ens& E = e->ens[id];
+    if (E.P.k_end > E.P.k0)
+      HIPCHK(launch_fill_u32(e->d_stage + E.P.stage_off, 0u, (E.P.k_end - E.P.k0) * E.n * E.ncomp, e->st));
+  }
+  if ((rc = kmc_pass(e, act, mode, false, rs, timed))) return rc;
+  for (size_t q = 0; q < act.size(); ++q) {
+    Ens& E = e->ens[act[q]];
+    const int rows = (int)(E.P.k_end - E.P.k0);
+    if (rows <= 0) continue;
+    if (eq_sampling(E.P.sample_mode))
+      HIPCHK(launch_corr_update(e->d_stage + E.P.stage_off, E.dD, E.dA, E.dC, e->ch.stall, E.chain0, E.n, E.p, E.m,
+                                E.s, rows, e->d_corr_cnt + cnt_off[q], E.P.sample_mode == kSampleMsd ? 1 : 0, e->st));
+    else
+      HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, E.n, rows, e->d_rows + row_off[q], e->st));
+  }
+  return DSM_OK;
+}
+
+// The window's launches captured into a graph (stream capture of the engine stream and the
+// streams a concurrent pass forks to), instantiated for replay.
+int capture_window(dsm_engine* e, const std::vector<int>& act, const std::vector<int64_t>& cnt_off,
+                   const std::vector<int64_t>& row_off, const std::vector<int>& cnt, dsm_run_stats* rs,
+                   hipGraphExec_t* exec) {
+  *exec = nullptr;
+  (void)rs;  // nothing runs during the capture: the replay is counted by the caller
+  dsm_run_stats none{};
+  HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+  e->capturing = true;
+  const int rc = window_launches(e, act, cnt_off, row_off, cnt, DSM_MODE_BATCHED, &none, nullptr);
+  e->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(e->st, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ce != hipSuccess || !g) return fail(DSM_ERR_DEVICE, "stream capture of a window failed: %s", hipGetErrorString(ce));
+  const hipError_t ie = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) return fail(DSM_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ie));
+  return DSM_OK;
+}
+
+void drop_graphs(dsm_engine* e) {
+  for (auto& c : e->graphs)
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+  e->graphs.clear();
+}
+
+// A captured window of this shape, captured now if there is none (at most kMaxGraphs kept).
+constexpr size_t kMaxGraphs = 16;
+int window_graph(dsm_engine* e, const std::vector<int64_t>& key, const std::vector<int>& act,
+                 const std::vector<int64_t>& cnt_off, const std::vector<int64_t>& row_off, const std::vector<int>& cnt,
+                 dsm_run_stats* rs, hipGraphExec_t* exec) {
+  for (auto& c : e->graphs)
+    if (c.key == key) {
+      c.used = ++e->graph_clock;
+      *exec = c.exec;
+      return DSM_OK;
+    }
+  // graphs of an older generation can never be replayed again
+  for (size_t i = 0; i < e->graphs.size();) {
+    if (e->graphs[i].key[0] != key[0]) {
+      (void)hipGraphExecDestroy(e->graphs[i].exec);
+      e->graphs.erase(e->graphs.begin() + (long)i);
+    } else {
+      ++i;
+    }
+  }
+  if (e->graphs.size() >= kMaxGraphs) {
+    auto lru = std::min_element(e->graphs.begin(), e->graphs.end(),
+                                [](const dsm_engine::Captured& a, const dsm_engine::Captured& b) { return a.used < b.used; });
+    (void)hipGraphExecDestroy(lru->exec);
+    e->graphs.erase(lru);
+  }
+  dsm_engine::Captured c;
+  c.key = key;
+  c.used = ++e->graph_clock;
+  int rc = capture_window(e, act, cnt_off, row_off, cnt, rs, &c.exec);
+  if (rc) return rc;
+  e->graphs.push_back(c);
+  *exec = c.exec;
   return DSM_OK;
 }
 
@@ -950,13 +1080,7 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
     }
     if (act.empty()) break;
     const bool last = std::all_of(reached.begin(), reached.end(), [](char r) { return r != 0; });
-    if ((rc = upload_ens(e))) return rc;
-    for (int id : act) {
-      Ens& E = e->ens[id];
-      if (E.P.k_end > E.P.k0)
-        HIPCHK(launch_fill_u32(e->d_stage + E.P.stage_off, 0u, (E.P.k_end - E.P.k0) * E.n * E.ncomp, e->st));
-    }
-    // the window's sample consumers: level counters uploaded before the pass
+    // the window's sample consumers: level counters set before the pass
     std::vector<int> cnt;
     std::vector<int64_t> cnt_off(act.size(), -1), row_off(act.size(), -1);
     int64_t flow_doubles = 0;
@@ -974,31 +1098,54 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
       }
     }
     if ((int)cnt.size() > e->d_corr_cnt_cap) {
+      e->gen++;
       if (e->d_corr_cnt) HIPCHK(hipFree(e->d_corr_cnt));
       if ((rc = dev_alloc(&e->d_corr_cnt, cnt.size()))) return rc;
       e->d_corr_cnt_cap = (int)cnt.size();
     }
     if (flow_doubles > e->d_rows_cap) {
+      e->gen++;
       if (e->d_rows) HIPCHK(hipFree(e->d_rows));
       if ((rc = dev_alloc(&e->d_rows, (size_t)flow_doubles))) return rc;
       e->d_rows_cap = flow_doubles;
     }
-    if (!cnt.empty())
-      HIPCHK(hipMemcpyAsync(e->d_corr_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, e->st));
     Pass timed;
-    if ((rc = kmc_pass(e, act, mode, false, &rs, &timed))) return rc;
-    // consume the window's samples: every reduction is queued, then one synchronisation
-    for (size_t q = 0; q < act.size(); ++q) {
-      Ens& E = e->ens[act[q]];
-      const int rows = (int)(E.P.k_end - E.P.k0);
-      if (rows <= 0) continue;
-      if (eq_sampling(E.P.sample_mode))
-        HIPCHK(launch_corr_update(e->d_stage + E.P.stage_off, E.dD, E.dA, E.dC, e->ch.stall, E.chain0, E.n, E.p, E.m,
-                                  E.s, rows, e->d_corr_cnt + cnt_off[q], E.P.sample_mode == kSampleMsd ? 1 : 0,
-                                  e->st));
-      else
-        HIPCHK(launch_flow_reduce(e->d_stage + E.P.stage_off, E.n, rows, e->d_rows + row_off[q], e->st));
+    bool graph_timed = false;
+    if (e->opt.graph && mode == DSM_MODE_BATCHED) {
+      // captured pass: window parameters through the pinned block, the launches replayed
+      const size_t need = sizeof(WinParam) * act.size() + sizeof(int) * cnt.size();
+      if (need > e->h_win_cap) {
+        e->gen++;
+        if (e->h_win) HIPCHK(hipHostFree(e->h_win));
+        e->h_win = nullptr;
+        e->h_win_cap = 0;
+        const size_t cap = std::max<size_t>(4096, 2 * need);
+        HIPCHK(hipHostMalloc(&e->h_win, cap, hipHostMallocCoherent));
+        e->h_win_cap = cap;
+      }
+      WinParam* wp = (WinParam*)e->h_win;
+      for (size_t q = 0; q < act.size(); ++q) {
+        const Ens& E = e->ens[act[q]];
+        wp[q] = WinParam{act[q], 0, E.P.t_end, E.P.k0, E.P.k_end};
+      }
+      if (!cnt.empty()) std::memcpy(wp + act.size(), cnt.data(), sizeof(int) * cnt.size());
+      std::vector<int64_t> key{(int64_t)e->gen};
+      for (int id : act) {
+        key.push_back(id);
+        key.push_back(e->ens[id].P.k_end - e->ens[id].P.k0);
+      }
+      hipGraphExec_t exec = nullptr;
+      if ((rc = window_graph(e, key, act, cnt_off, row_off, cnt, &rs, &exec))) return rc;
+      HIPCHK(hipEventRecord(e->evs[0], e->st));
+      HIPCHK(hipGraphLaunch(exec, e->st));
+      HIPCHK(hipEventRecord(e->evs[1], e->st));
+      graph_timed = true;
+      rs.kmc_launches++;
+      rs.first_launches++;
+    } else {
+      if ((rc = window_launches(e, act, cnt_off, row_off, cnt, mode, &rs, &timed))) return rc;
     }
+    // consume the window's samples: every reduction is queued, then one synchronisation
     std::vector<double> flow_host((size_t)flow_doubles);
     if (flow_doubles)
       HIPCHK(hipMemcpyAsync(flow_host.data(), e->d_rows, sizeof(double) * flow_doubles, hipMemcpyDeviceToHost,
@@ -1012,6 +1159,12 @@ int advance_impl(dsm_engine* e, const std::vector<int>& ids, const std::vector<d
     }
     HIPCHK(hipStreamSynchronize(e->st));
     if ((rc = pass_times(e, timed, &rs))) return rc;
+    if (graph_timed) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, e->evs[0], e->evs[1]));
+      rs.kmc_ms += ms;
+      rs.first_kmc_ms += ms;
+    }
     for (size_t q = 0; q < act.size(); ++q) {
       Ens& E = e->ens[act[q]];
       const int64_t rows = E.P.k_end - E.P.k0;
@@ -1109,6 +1262,8 @@ int dsm_destroy(dsm_engine* e) {
   if (e->st) (void)hipStreamSynchronize(e->st);
   for (auto& x : e->xs)
     if (x) (void)hipStreamSynchronize(x);
+  drop_graphs(e);
+  if (e->h_win) (void)hipHostFree(e->h_win);
   for (auto& E : e->ens) {
     if (E.dD) (void)hipFree(E.dD);
     if (E.dA) (void)hipFree(E.dA);
@@ -1144,6 +1299,7 @@ int dsm_add_ensemble(dsm_engine* e, const dsm_ensemble_desc* d, int32_t* ens_id)
   if ((d->sample_mode != DSM_SAMPLE_NONE) && !(d->delta > 0.0)) return fail(DSM_ERR_ARG, "delta must be positive");
   int rc = set_device(e);
   if (rc) return rc;
+  e->gen++;
   Ens E;
   E.d = *d;
   E.d.nk_per_chain = nullptr;
@@ -1292,6 +1448,7 @@ int dsm_remove_ensemble(dsm_engine* e, int32_t ens) {
   Ens* E = get_ens(e, ens);
   if (!E) return fail(DSM_ERR_ARG, "no ensemble %d", ens);
   (void)set_device(e);
+  e->gen++;
   if (E->dD) { (void)hipFree(E->dD); (void)hipFree(E->dA); (void)hipFree(E->dC); E->dD = E->dA = E->dC = nullptr; }
   E->live = false;
   E->flow_rows.clear();
@@ -1530,6 +1687,7 @@ int dsm_stress(dsm_engine* e, int32_t ens, double* out, int32_t* healthy) {
 int dsm_set_option(dsm_engine* e, int32_t key, int32_t value) {
   if (!e) return fail(DSM_ERR_ARG, "null engine");
   EngineOptions& o = e->opt;
+  e->gen++;  // captured passes hold the options they were captured with
   switch (key) {
     case DSM_OPT_PLAIN_JUMPS: if (value < 1) break; o.plain_jumps = value; return DSM_OK;
     case DSM_OPT_SMALL_PER_CU: if (value < 0) break; o.small_per_cu = value; return DSM_OK;
@@ -1538,6 +1696,7 @@ int dsm_set_option(dsm_engine* e, int32_t key, int32_t value) {
     case DSM_OPT_CONCURRENT: if (value < -1 || value > 1) break; o.concurrent = value; return DSM_OK;
     case DSM_OPT_BLOCKS_PER_CU: if (value < 0) break; o.blocks_per_cu = value; return DSM_OK;
     case DSM_OPT_WINDOW: if (value < 1) break; o.window = value; return DSM_OK;
+    case DSM_OPT_GRAPH: if (value < 0 || value > 1) break; o.graph = value; return DSM_OK;
     default: return fail(DSM_ERR_ARG, "unknown option %d", key);
   }
   return fail(DSM_ERR_ARG, "option %d: value %d out of range", key, value);
@@ -1554,6 +1713,7 @@ int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value) {
     case DSM_OPT_CONCURRENT: *value = o.concurrent; return DSM_OK;
     case DSM_OPT_BLOCKS_PER_CU: *value = o.blocks_per_cu; return DSM_OK;
     case DSM_OPT_WINDOW: *value = o.window; return DSM_OK;
+    case DSM_OPT_GRAPH: *value = o.graph; return DSM_OK;
     default: return fail(DSM_ERR_ARG, "unknown option %d", key);
   }
 }

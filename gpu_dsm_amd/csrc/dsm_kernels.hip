@@ -519,6 +519,23 @@ __global__ void __launch_bounds__(256) corr_reduce_kernel(const double* C, int l
   if (threadIdx.x == 0) { out_sum[pt] = red[0]; if (pt == 0) *out_healthy = cnt[0]; }
 }
 
+// ------------------------------------------------------------------ captured passes
+// Head of a captured window pass: the window's parameters, which change every window,
+// come from a block of pinned host memory the host fills before each replay, so the
+// graph itself never changes: (t_end, k0, k_end) of every ensemble of the pass into its
+// EnsParams, then the correlator level counters.  One block.
+__global__ void __launch_bounds__(256) stage_window_kernel(const WinParam* __restrict__ wp, int n_ens,
+                                                           EnsParams* P, const int* __restrict__ cnt_src,
+                                                           int* cnt_dst, int n_cnt) {
+  for (int i = threadIdx.x; i < n_ens; i += blockDim.x) {
+    const WinParam w = wp[i];
+    P[w.ens].t_end = w.t_end;
+    P[w.ens].k0 = w.k0;
+    P[w.ens].k_end = w.k_end;
+  }
+  for (int i = threadIdx.x; i < n_cnt; i += blockDim.x) cnt_dst[i] = cnt_src[i];
+}
+
 // ------------------------------------------------------------------ ensemble stress now
 // Stress of the current state (common/ensemble_kernel.cu:504-538 per chain, averaged by
 // common/ensemble.cu:210-221): per chain sigma_ab = -sum_i 3 Q_ia Q_ib / N_i, Lpp =
@@ -842,6 +859,12 @@ hipError_t launch_corr_reduce(const double* C, int ld, const int* stall, int cha
   if (npts <= 0) return hipSuccess;
   hipLaunchKernelGGL(corr_reduce_kernel, dim3(npts), dim3(256), 0, st, C, ld, stall, chain0, first, count, npts,
                      out_sum, out_healthy);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_window(const WinParam* wp, int n_ens, EnsParams* P, const int* cnt_src, int* cnt_dst,
+                               int n_cnt, hipStream_t st) {
+  hipLaunchKernelGGL(stage_window_kernel, dim3(1), dim3(256), 0, st, wp, n_ens, P, cnt_src, cnt_dst, n_cnt);
   return hipGetLastError();
 }
 

@@ -1362,4 +1362,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW ==
   }
 }
 
+// Row and pair forms in one launch (concurrent passes): each wave takes chains from the
+// row-form list until it is exhausted, then from the pair-form list.  Both shapes run
+// within the same register budget (three waves per SIMD) and the same staging size, so
+// the merged launch costs neither occupancy nor LDS, and it leaves the pass's other
+// hardware queues to the wave-form geometries: the long-chain runs, which set a window's
+// critical path, and the wave-form bulk then all start at once.  Each wave restages and
+// reloads its rate tables when it moves to the pair list, so every chain runs exactly
+// as in the separate launches.
+template <bool LT, bool EXT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) kmc_rowpair_kernel(KmcArgs row,
+                                                                                                 KmcArgs pair) {
+  static_assert(stage_floats<4, 1>() == stage_floats<2, 1>(), "row and pair forms stage the same floats");
+  extern __shared__ float lds_dyn[];
+  lds_f* stg = (lds_f*)lds_dyn;
+  lds_f* ltab = stg + stage_floats<4, 1>();
+  run_stream<4, 1, LT, EXT>(row, stg, ltab);
+  run_stream<2, 1, LT, EXT>(pair, stg, ltab);
+}
+
 }  // namespace dsm

@@ -1,9 +1,10 @@
 // dsm_kmc_inst.hip -- instantiates the KMC kernels for one chain geometry.
-// Built once per geometry (-DDSM_GEOM=0..5) so the variants compile in parallel:
+// Built once per geometry (-DDSM_GEOM=0..6) so the variants compile in parallel:
 //   0: row form, four chains per wave, Z <= 64
 //   1: pair form, two chains per wave, Z <= 128
 //   2, 3, 4, 5: wave form, one chain per wave, Z <= 256, 512, 1024, 2048 (the last keeps
 //   part of its state in scratch memory; it carries the rare chains with Z > 1024)
+//   6: the row and pair forms in one launch (batched, concurrent passes)
 #include "dsm_kmc.h"
 #include "dsm_launch.h"
 
@@ -14,6 +15,42 @@
 #define DSM_CAT(a, b) DSM_CAT2(a, b)
 
 namespace dsm {
+
+#if DSM_GEOM == 6
+namespace {
+constexpr int kMaxLdsFloats = 65536 / 4;
+bool tables_fit(int lds_floats) { return lds_floats > 0 && stage_floats<4, 1>() + lds_floats <= kMaxLdsFloats; }
+
+template <bool LT, bool EXT>
+hipError_t launch_rp(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st) {
+  const size_t lds = sizeof(float) * ((size_t)stage_floats<4, 1>() + (LT ? (size_t)row.lds_floats : 0));
+  hipLaunchKernelGGL((kmc_rowpair_kernel<LT, EXT>), dim3(grid), dim3(64), lds, st, row, pair);
+  return hipGetLastError();
+}
+
+template <bool LT, bool EXT>
+int occupancy_rp(int lds_floats) {
+  int nb = 0;
+  const size_t lds = sizeof(float) * ((size_t)stage_floats<4, 1>() + (LT ? (size_t)lds_floats : 0));
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_rowpair_kernel<LT, EXT>, 64, lds);
+  return e == hipSuccess ? nb : 0;
+}
+}  // namespace
+
+hipError_t launch_kmc_rowpair(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st) {
+  if (grid <= 0) return hipSuccess;
+  const bool lt = tables_fit(row.lds_floats);
+  if (row.ext) return lt ? launch_rp<true, true>(row, pair, grid, st) : launch_rp<false, true>(row, pair, grid, st);
+  return lt ? launch_rp<true, false>(row, pair, grid, st) : launch_rp<false, false>(row, pair, grid, st);
+}
+
+int kmc_rowpair_occupancy(int lds_floats, bool ext) {
+  const bool lt = tables_fit(lds_floats);
+  if (ext) return lt ? occupancy_rp<true, true>(lds_floats) : occupancy_rp<false, true>(0);
+  return lt ? occupancy_rp<true, false>(lds_floats) : occupancy_rp<false, false>(0);
+}
+
+#else
 
 namespace {
 constexpr int kCPW = DSM_GEOM == 0 ? 4 : (DSM_GEOM == 1 ? 2 : 1);
@@ -63,5 +100,6 @@ hipError_t DSM_CAT(launch_kmc_geom_, DSM_GEOM)(const KmcArgs& a, bool plain, int
 int DSM_CAT(kmc_occupancy_geom_, DSM_GEOM)(bool plain, int lds_floats, bool ext) {
   return ext ? occupancy_variant<true>(plain, lds_floats) : occupancy_variant<false>(plain, lds_floats);
 }
+#endif
 
 }  // namespace dsm

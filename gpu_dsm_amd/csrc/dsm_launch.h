@@ -27,6 +27,9 @@ DSM_DECLARE_GEOM(3)
 DSM_DECLARE_GEOM(4)
 DSM_DECLARE_GEOM(5)
 #undef DSM_DECLARE_GEOM
+// the row form (list `row`) and the pair form (list `pair`) in one batched launch
+hipError_t launch_kmc_rowpair(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st);
+int kmc_rowpair_occupancy(int lds_floats, bool ext);
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const float* tabs, const int* chains, int n,
                           hipStream_t st);
 hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st);
@@ -58,6 +61,10 @@ struct KappaArg {
 };
 hipError_t launch_chain_stress(const ChainArrays& ch, int chain0, int n, const float kappa[9], double* part,
                                hipStream_t st);
+// window parameters (t_end, k0, k_end per ensemble) and correlator counters of a captured
+// pass, from host-pinned memory into EnsParams and cnt_dst
+hipError_t launch_stage_window(const WinParam* wp, int n_ens, EnsParams* P, const int* cnt_src, int* cnt_dst,
+                               int n_cnt, hipStream_t st);
 // invariant violations of chains chain0..chain0+n-1 added to out[0..6] (see dsm_check_state)
 hipError_t launch_check_state(const ChainArrays& ch, int chain0, int n, double t_min, unsigned long long* out,
                               hipStream_t st);

@@ -49,6 +49,16 @@ struct EnsParams {
   int64_t stage_off; // float offset of this ensemble's staging rows
 };
 
+// Window parameters of one ensemble in a captured pass (written by the host into
+// pinned memory before each replay; stage_window_kernel copies them into EnsParams)
+struct WinParam {
+  int ens;
+  int pad;
+  double t_end;
+  int64_t k0;
+  int64_t k_end;
+};
+
 struct ChainArrays {
   int* Z;
   int* nk;

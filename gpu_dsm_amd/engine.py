@@ -27,7 +27,7 @@ EXPORTS = ["dsm_create", "dsm_destroy", "dsm_last_error", "dsm_mem_info", "dsm_a
            "dsm_ensemble_info", "dsm_save", "dsm_load", "dsm_load_error", "dsm_stress", "dsm_check_state",
            "dsm_serve_ex", "dsm_correlator_range", "dsm_set_option", "dsm_get_option", "dsm_stress_noalloc"]
 OPTIONS = {"plain_jumps": 1, "small_per_cu": 2, "split_margin": 3, "lds_table_floats": 4, "concurrent": 5,
-           "blocks_per_cu": 6, "window": 7}
+           "blocks_per_cu": 6, "window": 7, "graph": 8}
 CHECKS = ["z_range", "n_below_1", "n_sum", "end_q", "nonfinite", "stalled", "behind"]
 
 

@@ -104,6 +104,8 @@ int dsm_mem_info(int32_t device_id, size_t* free_bytes, size_t* total_bytes);
 #define DSM_OPT_CONCURRENT 5        /* geometries side by side: -1 by the pass's lengths, 0 never, 1 always */
 #define DSM_OPT_BLOCKS_PER_CU 6     /* cap on persistent waves per CU (0 = the occupancy limit) */
 #define DSM_OPT_WINDOW 7            /* samples per staging window of ensembles added later (256) */
+#define DSM_OPT_GRAPH 8             /* 1: batched windows run as captured HIP graphs, replayed with the
+                                       window's parameters in a pinned block (0) */
 int dsm_set_option(dsm_engine* e, int32_t key, int32_t value);
 int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value);
 

@@ -20,4 +20,5 @@ def test_tune_small_workload(tmp_path):
     assert again["cached"] and again["best"] == res["best"]
     with g.Engine(0) as e:
         A.apply(e, res["best"])
-        assert e.options() == dict(res["best"], plain_jumps=1)
+        opts = e.options()
+        assert all(opts[k] == v for k, v in res["best"].items())
