@@ -752,30 +752,41 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   HIPCHK(launch_split_geoms(list(0), ps.n0, e->ch.Z, e->ch.stall, list(kSplitList), (int)cap, cnt + kSplitList,
                             e->opt.split_margin_div, e->st));
   HIPCHK(hipEventRecord(e->ev_split, e->st));
-  // first launches.  A process has four hardware queues on this device, so four streams:
-  // the row and pair forms in one merged launch on the engine stream; the longest chains
-  // (G = 8: few chains, a latency-bound run that sets the window's critical path) alone
-  // on one; the wave-form bulk of G = 1 on another; G = 4 (latency-bound) followed by
-  // the G = 2 bulk on the last.  Every geometry thus starts at the split.  (With the row
-  // and pair forms on separate queues, the G = 1 bulk had to wait for the G = 8 run: on
-  // the polydisperse melt a 16 tau_K window took 124-152 ms, its G = 8 kernel 67-98 ms
-  // of it.)
-  static const int kStreamOf[kNumBins] = {0, 0, 3, 1, 1, 2};
-  for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
+  // first launches.  A process has four hardware queues on this device, so four streams.
+  // The longest chains (G = 8: few chains, a latency-bound run that sets the window's
+  // critical path) go first, on the engine stream right behind the split, so their
+  // waves -- each needs a whole SIMD's registers -- are dispatched before the other
+  // queues, released by the split's event, fill the device (launched on a side stream
+  // they waited for a free SIMD: 140-180 ms for a 68 ms run).  G = 4 (also latency-bound)
+  // then the G = 2 bulk on the second stream, the merged row and pair forms on the
+  // third, the G = 1 bulk on the fourth.  The engine stream then runs a second launch of
+  // the G = 2 and the G = 1 bins: it shares their work queues, so whichever stream is
+  // free first takes the remaining chains of those bins (a launch that finds its queue
+  // empty exits at once).
   auto stream = [&](int k) { return k == 0 ? e->st : e->xs[k]; };  // xs[0] is not a stream
-  for (int b = ps.bin_max; b >= 2; --b) {
+  auto first = [&](int b, int k) -> hipError_t {
+    if (b > ps.bin_max) return hipSuccess;
     bind(kSplitList + b, late_of(b));
-    HIPCHK(launch_kmc(a, b, false, grid(b), stream(kStreamOf[b])));
-  }
-  for (int k = 1; k < 4; ++k) HIPCHK(hipEventRecord(e->ev_bin[k], e->xs[k]));
+    ++rs->kmc_launches;
+    return launch_kmc(a, b, false, grid(b), stream(k));
+  };
+  HIPCHK(first(5, 0));
+  for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
+  HIPCHK(first(4, 1));
+  HIPCHK(first(3, 1));
+  HIPCHK(first(2, 3));
   bind(kSplitList + 1, late_of(1));
   const KmcArgs pair_args = a;
   bind(kSplitList, late_of(0));
   int nb = kmc_rowpair_occupancy(a.lds_floats, a.ext != 0);
   if (e->opt.blocks_per_cu > 0) nb = std::min(nb, e->opt.blocks_per_cu);
-  if (timed) HIPCHK(hipEventRecord(e->cev[1], e->st));
-  HIPCHK(launch_kmc_rowpair(a, pair_args, std::max(nb, 1) * e->n_cu, e->st));
-  if (timed) HIPCHK(hipEventRecord(e->cev[2], e->st));
+  if (timed) HIPCHK(hipEventRecord(e->cev[1], e->xs[2]));
+  HIPCHK(launch_kmc_rowpair(a, pair_args, std::max(nb, 1) * e->n_cu, e->xs[2]));
+  if (timed) HIPCHK(hipEventRecord(e->cev[2], e->xs[2]));
+  ++rs->kmc_launches;
+  HIPCHK(first(3, 0));
+  HIPCHK(first(2, 0));
+  for (int k = 1; k < 4; ++k) HIPCHK(hipEventRecord(e->ev_bin[k], e->xs[k]));
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[k], 0));
   // late launches: chains that outgrew their geometry, through the larger ones in order
   for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) {
@@ -784,7 +795,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   }
   if (timed) HIPCHK(hipEventRecord(e->cev[3], e->st));
   ps.pending = timed ? 2 : 0;  // read by pass_times after the caller's next synchronisation
-  rs->kmc_launches += ps.bin_max + (late_last(ps.bin_max) - kLateFirst + 1);
+  rs->kmc_launches += late_last(ps.bin_max) - kLateFirst + 1;
   rs->first_launches++;
   return DSM_OK;
 }

@@ -1344,6 +1344,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW ==
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<CPW, G>();
+  // G >= 4: few long chains, each a serial run of thousands of jumps per window that
+  // sets the pass's critical path.  Their waves take the SIMD's issue slot ahead of the
+  // co-resident row, pair and G <= 2 waves, which are throughput work and fill the
+  // cycles these leave idle.
+  if constexpr (CPW == 1 && G >= 4 && !PLAIN) __builtin_amdgcn_s_setprio(3);
   if constexpr (!PLAIN && CPW > 1) {
     run_stream<CPW, G, LT, EXT>(a, stg, ltab);
   } else {

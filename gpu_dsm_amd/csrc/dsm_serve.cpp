@@ -128,6 +128,7 @@ extern "C" int dsm_serve_ex(int32_t device_id, const dsm_serve_config* cfg, dsm_
   dsm_engine* e = nullptr;
   int rc = dsm_create(device_id, &e);
   if (rc) return rc;
+  if (cfg->graph && (rc = dsm_set_option(e, DSM_OPT_GRAPH, 1))) { dsm_destroy(e); return rc; }
   size_t free_b = 0, total_b = 0;
   if ((rc = dsm_mem_info(device_id, &free_b, &total_b))) { dsm_destroy(e); return rc; }
   const int64_t budget = (int64_t)((double)free_b * cfg->hbm_fraction);

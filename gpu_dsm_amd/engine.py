@@ -60,7 +60,7 @@ class ServeConfig(C.Structure):
                 ("beta", C.c_float * 4), ("nch_min", C.c_int32), ("nch_max", C.c_int32),
                 ("flow_fraction", C.c_double), ("rate_min", C.c_double), ("rate_max", C.c_double),
                 ("horizon", C.c_double), ("window", C.c_double), ("cd", C.c_int32), ("hbm_fraction", C.c_double),
-                ("progress_s", C.c_double)]
+                ("progress_s", C.c_double), ("graph", C.c_int32)]
 
 
 class ServeReport(C.Structure):
@@ -80,7 +80,7 @@ class ServeReport(C.Structure):
 SERVE_DEFAULTS = dict(n_requests=2000, arrival_rate=20.0, seed=1, nk_min=16, nk_max=512, nk_step=16,
                       beta=(0.5, 1.0, 2.0), nch_min=1024, nch_max=65536, flow_fraction=0.5, rate_min=1e-5,
                       rate_max=1e-2, horizon=1.0e5, window=10.0, cd=1, hbm_fraction=0.8,
-                      progress_s=0.0)
+                      progress_s=0.0, graph=0)
 
 
 class DeviceChains(C.Structure):

@@ -283,6 +283,7 @@ typedef struct {
   int32_t cd;                /* constraint dynamics */
   double hbm_fraction;       /* share of the free device memory admission may fill */
   double progress_s;         /* > 0: one status line on stderr every progress_s seconds */
+  int32_t graph;             /* 1: scheduling rounds run as captured graphs (DSM_OPT_GRAPH) */
 } dsm_serve_config;
 
 typedef struct {

@@ -35,6 +35,7 @@ static int fake_fail(int rc, const char* m) {
 
 extern "C" {
 const char* dsm_last_error(void) { return g_fake_err.c_str(); }
+int dsm_set_option(dsm_engine* e, int32_t, int32_t) { return e ? DSM_OK : DSM_ERR_ARG; }
 int dsm_mem_info(int32_t, size_t* f, size_t* t) {
   if (f) *f = (size_t)64 << 30;
   if (t) *t = (size_t)288 << 30;
