@@ -4,14 +4,16 @@
     python -m gpu_dsm_amd.workloads c3 [--horizon T]      262,144 chains, log-normal Nk, CD on, G(t)
     python -m gpu_dsm_amd.workloads c4 [--horizon T]      16 sheared ensembles x 16384 chains, Nk=50
     python -m gpu_dsm_amd.workloads c5 [--horizon T]      serving stream, 2000 requests at 20/s
+    python -m gpu_dsm_amd.workloads eta0 [--horizon T]    Green-Kubo viscosity of the C4 chains at rest
 
-The headline workload (131,072 chains, Nk=100, CD on, G(t)) is bench.py at the repo
-root.  Every run starts from the equilibrium draw; all inputs are synthetic
+bench.py at the repo root measures the headline (the c3 melt, 16 tau_K windows) and the
+131,072-chain Nk=100 workload as its secondary line.  Every run starts from the equilibrium draw; all inputs are synthetic
 (Philox-keyed); times are wall clock around the device work.
 """
 import argparse
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -72,9 +74,13 @@ def c3(horizon=100.0, seed=1, n=262144):
                 chains_Z_over_64=int((z1 > 64).sum()), G0=float(g[0]))
 
 
-def c4(horizon=200.0, seed=1):
+def c4(horizon=1.0e5, seed=1, segment=5000.0):
     """16 ensembles x 16384 chains, Nk=50, beta=1, CD on, shear rates 1e-5..1e-1 per
-    tau_K (log-spaced), all advanced in one batched pass per output window."""
+    tau_K (log-spaced), all advanced in one batched pass per output window.  Runs in
+    segments of `segment` tau_K (a progress line on stderr after each).  The steady-state
+    stress is the mean over the second half of the run with a standard error from ten
+    consecutive blocks; a rate counts as steady when the discarded first half holds at
+    least 10 strain units (rate * horizon / 2 >= 10)."""
     n, nk, k = 16384, 50, 16
     rates = [10.0 ** (-5.0 + 4.0 * q / (k - 1)) for q in range(k)]
     with eng.Engine(0) as e:
@@ -85,20 +91,55 @@ def c4(horizon=200.0, seed=1):
             e.init_equilibrium(ens)
             ids.append(ens)
         t0 = time.perf_counter()
-        st = e.advance_many(ids, horizon)
+        jumps, t = 0, 0.0
+        while t < horizon:
+            t = min(horizon, t + segment)
+            st = e.advance_many(ids, t)
+            jumps += st["events"]
+            print("c4: t = %g tau_K, %.3g jumps, %.1f s" % (t, jumps, time.perf_counter() - t0), file=sys.stderr,
+                  flush=True)
         wall = time.perf_counter() - t0
         out = []
         for ens, gd in zip(ids, rates):
             rows = e.flow_stress(ens)
             half = rows[rows.shape[0] // 2:]
-            sxy = float(np.mean(half[:, 3] / half[:, 8]))
-            n1 = float(np.mean((half[:, 0] - half[:, 1]) / half[:, 8]))
-            out.append(dict(rate=gd, sigma_xy=sxy, N1=n1))
+            sxy_t = half[:, 3] / half[:, 8]
+            n1_t = (half[:, 0] - half[:, 1]) / half[:, 8]
+            blocks = np.array_split(np.arange(half.shape[0]), 10)
+            bs = np.array([sxy_t[b].mean() for b in blocks])
+            bn = np.array([n1_t[b].mean() for b in blocks])
+            out.append(dict(rate=gd, sigma_xy=float(sxy_t.mean()), sigma_xy_se=float(bs.std(ddof=1) / np.sqrt(10)),
+                            N1=float(n1_t.mean()), N1_se=float(bn.std(ddof=1) / np.sqrt(10)),
+                            viscosity=float(-sxy_t.mean() / gd), steady=bool(gd * horizon / 2 >= 10.0)))
     return dict(workload="c4: 16 x 16384 chains, Nk=50, beta=1, CD on, shear 1e-5..1e-1", horizon_tau_k=horizon,
-                jumps=st["events"], seconds=wall, jumps_per_s=st["events"] / wall,
-                note="stress averages over the second half of a short run (not steady state at low rates); "
-                     "sign convention of the reference's output: sigma = -sum 3 Q Q / N per chain",
+                jumps=jumps, seconds=wall, jumps_per_s=jumps / wall,
+                note="means over the second half of the run, standard errors from ten blocks; sign convention "
+                     "of the reference's output: sigma = -sum 3 Q Q / N per chain, so sigma_xy < 0 under "
+                     "positive shear and viscosity = -sigma_xy / rate",
                 per_rate=out)
+
+
+def eta0(horizon=1.0e6, seed=7, n=16384, nk=50):
+    """Zero-shear viscosity of the C4 chains at rest by Green-Kubo: eta0 = int_0^inf G(t)
+    dt (trapezoid over the multi-tau lags), G(t) the stress autocorrelation of the
+    equilibrium run; the standard error is over 16 groups of chains.  Steady shear at a
+    rate well below 1/tau_d gives sigma_xy = -rate * eta0 (reference sign convention)."""
+    groups = 16
+    with eng.Engine(0) as e:
+        ens = e.add_ensemble(n, nk, 1.0, 1, seed=seed, sample="gt", delta=1.0, corr_p=16, corr_m=2, corr_levels=18)
+        e.init_equilibrium(ens)
+        t0 = time.perf_counter()
+        st = e.advance(ens, horizon)
+        wall = time.perf_counter() - t0
+        lag, g = e.correlator(ens)
+        _, gg = e.correlator_chains(ens, 0, n, n // groups)
+    keep = lag <= horizon / 10.0  # lags with at least ten independent windows behind them
+    trap = lambda y: float(np.sum(0.5 * (y[1:] + y[:-1]) * np.diff(lag[keep])))
+    per = np.array([trap(x[keep]) for x in gg])
+    return dict(workload="eta0: %d chains, Nk=%d, beta=1, CD on, G(t) at rest" % (n, nk), horizon_tau_k=horizon,
+                jumps=st["events"], seconds=wall, G0=float(g[0]), eta0=trap(g[keep]),
+                eta0_se=float(per.std(ddof=1) / np.sqrt(groups)), max_lag=float(lag[keep].max()),
+                G_at_max_lag_over_G0=float(g[keep][-1] / g[0]))
 
 
 def c5(horizon=20.0, window=5.0, seed=1, n_requests=2000, arrival_rate=20.0):
@@ -110,18 +151,21 @@ def c5(horizon=20.0, window=5.0, seed=1, n_requests=2000, arrival_rate=20.0):
     rep.update(workload="c5: serving stream, %d requests at %g/s" % (n_requests, arrival_rate),
                horizon_tau_k=horizon, window_tau_k=window,
                offered_chains=int(plan["n_chains"].sum()),
-               note="scaled horizon: the stated 1e5 tau_K per request is about 1.7e14 jumps in total")
+               device_busy=rep["advance_s"] / rep["makespan_s"],
+               note="scaled horizon: the stated 1e5 tau_K per request is about 1.7e14 jumps in total; "
+                    "device_busy = time in KMC rounds / makespan (near 1: the device, not the arrivals, "
+                    "sets the pace)")
     return rep
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["c1", "c3", "c4", "c5"])
+    ap.add_argument("which", choices=["c1", "c3", "c4", "c5", "eta0"])
     ap.add_argument("--horizon", type=float, default=None)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--requests", type=int, default=None, help="c5: number of requests")
     args = ap.parse_args()
-    fn = dict(c1=c1, c3=c3, c4=c4, c5=c5)[args.which]
+    fn = dict(c1=c1, c3=c3, c4=c4, c5=c5, eta0=eta0)[args.which]
     kw = dict(seed=args.seed)
     if args.horizon is not None:
         kw["horizon"] = args.horizon
