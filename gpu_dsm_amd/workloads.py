@@ -21,12 +21,16 @@ import numpy as np
 from . import engine as eng
 
 
-def c1(horizon=1.0e5, seed=1):
-    """1000 chains, Nk=25, beta=1, CD off, G(t) with p=16, m=2 (the small CLI case)."""
+def c1(horizon=1.0e5, seed=1, window=256, graph=0):
+    """1000 chains, Nk=25, beta=1, CD off, G(t) with p=16, m=2 (the small CLI case).
+    `window`: samples (tau_K) per staging window, the host synchronisation interval;
+    `graph`: 1 runs the windows as captured HIP graphs."""
     n, nk, p, m = 1000, 25, 16, 2
     s = int(math.ceil(math.log(horizon / p) / math.log(m))) + 1
     with eng.Engine(0) as e:
-        ens = e.add_ensemble(n, nk, 1.0, 0, seed=seed, sample="gt", delta=1.0, corr_p=p, corr_m=m, corr_levels=s)
+        e.set_option("graph", graph)
+        ens = e.add_ensemble(n, nk, 1.0, 0, seed=seed, sample="gt", delta=1.0, corr_p=p, corr_m=m, corr_levels=s,
+                             window=window)
         e.init_equilibrium(ens)
         z0 = e.state(ens)["Z"]
         t0 = time.perf_counter()
@@ -34,6 +38,7 @@ def c1(horizon=1.0e5, seed=1):
         wall = time.perf_counter() - t0
         lag, g = e.correlator(ens)
     return dict(workload="c1: 1000 chains, Nk=25, beta=1, CD off, G(t) p=16 m=2", horizon_tau_k=horizon,
+                window=window, graph=graph,
                 jumps=st["events"], seconds=wall, jumps_per_s=st["events"] / wall,
                 G0=float(g[0]), G0_from_initial_Z=float(np.mean(np.maximum(z0 - 2, 0))),
                 npoints=int(len(g)))
@@ -129,7 +134,12 @@ def eta0(horizon=1.0e6, seed=7, n=16384, nk=50):
         ens = e.add_ensemble(n, nk, 1.0, 1, seed=seed, sample="gt", delta=1.0, corr_p=16, corr_m=2, corr_levels=18)
         e.init_equilibrium(ens)
         t0 = time.perf_counter()
-        st = e.advance(ens, horizon)
+        jumps, t = 0, 0.0
+        while t < horizon:
+            t = min(horizon, t + horizon / 10.0)
+            jumps += e.advance(ens, t)["events"]
+            print("eta0: t = %g tau_K, %.3g jumps, %.1f s" % (t, jumps, time.perf_counter() - t0), file=sys.stderr,
+                  flush=True)
         wall = time.perf_counter() - t0
         lag, g = e.correlator(ens)
         _, gg = e.correlator_chains(ens, 0, n, n // groups)
@@ -137,16 +147,16 @@ def eta0(horizon=1.0e6, seed=7, n=16384, nk=50):
     trap = lambda y: float(np.sum(0.5 * (y[1:] + y[:-1]) * np.diff(lag[keep])))
     per = np.array([trap(x[keep]) for x in gg])
     return dict(workload="eta0: %d chains, Nk=%d, beta=1, CD on, G(t) at rest" % (n, nk), horizon_tau_k=horizon,
-                jumps=st["events"], seconds=wall, G0=float(g[0]), eta0=trap(g[keep]),
+                jumps=jumps, seconds=wall, G0=float(g[0]), eta0=trap(g[keep]),
                 eta0_se=float(per.std(ddof=1) / np.sqrt(groups)), max_lag=float(lag[keep].max()),
                 G_at_max_lag_over_G0=float(g[keep][-1] / g[0]))
 
 
-def c5(horizon=20.0, window=5.0, seed=1, n_requests=2000, arrival_rate=20.0):
+def c5(horizon=20.0, window=5.0, seed=1, n_requests=2000, arrival_rate=20.0, graph=0):
     """Serving stream: 2000 requests at 20/s, Nk 16..512, beta {0.5,1,2}, 1024..65536
     chains, half sheared; `horizon` tau_K per request (scaled; see the note)."""
     rep, lat = eng.serve(horizon=horizon, window=window, seed=seed, n_requests=n_requests,
-                         arrival_rate=arrival_rate, progress_s=10.0)
+                         arrival_rate=arrival_rate, progress_s=10.0, graph=graph)
     plan = eng.serve_plan(horizon=horizon, seed=seed, n_requests=n_requests, arrival_rate=arrival_rate)
     rep.update(workload="c5: serving stream, %d requests at %g/s" % (n_requests, arrival_rate),
                horizon_tau_k=horizon, window_tau_k=window,
@@ -164,6 +174,8 @@ def main():
     ap.add_argument("--horizon", type=float, default=None)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--requests", type=int, default=None, help="c5: number of requests")
+    ap.add_argument("--window", type=int, default=None, help="c1: samples per staging window")
+    ap.add_argument("--graph", type=int, default=None, help="c1, c5: 1 = windows as captured HIP graphs")
     args = ap.parse_args()
     fn = dict(c1=c1, c3=c3, c4=c4, c5=c5, eta0=eta0)[args.which]
     kw = dict(seed=args.seed)
@@ -171,6 +183,10 @@ def main():
         kw["horizon"] = args.horizon
     if args.requests is not None and args.which == "c5":
         kw["n_requests"] = args.requests
+    if args.window is not None and args.which == "c1":
+        kw["window"] = args.window
+    if args.graph is not None and args.which in ("c1", "c5"):
+        kw["graph"] = args.graph
     print(json.dumps(fn(**kw)), flush=True)
 
 
