@@ -35,6 +35,10 @@ Printed: one JSON line.
   gt_err_vs_fp64  1000 chains, Nk 25, CD off, to 1e5 tau_K (the small configuration in full):
                device G(t) against the FP64 oracle, paired per-chain differences with
                standard errors, over the lags where G >= 1% of G(0).
+  dist_err_vs_fp64  the Z / N / Q distributions of the first 4096 chains of the headline
+               ensemble (own melt partners) after 100 tau_K: device against the FP64
+               oracle on the same chains and Philox streams (largest pmf / CDF
+               differences), and each against the equilibrium laws in sampling-error units.
 """
 import argparse
 import json
@@ -283,6 +287,36 @@ def gt_error(g, args):
             "fp64_oracle_seconds": sec64}
 
 
+def dist_error(g, args, nk_full, n=4096, T=100.0):
+    """Z / N / Q distributions of the same chains after T tau_K: device vs FP64 oracle,
+    and each against the equilibrium laws (gpu_dsm_amd.distributions)."""
+    from oracle import oracle as O
+    from gpu_dsm_amd import distributions as D
+    nk = np.ascontiguousarray(nk_full[:n], np.int32)
+    nmax = int(nk.max())
+    with g.Engine(0) as eng:
+        ens = eng.add_ensemble(n, nmax, C3["beta"], C3["cd"], seed=args.seed, sample="none", nk_per_chain=nk,
+                               partner="melt")
+        eng.init_equilibrium(ens)
+        eng.advance(ens, T)
+        dev = D.fetch_zn_q(eng, ens)
+    P = O.Params(C3["beta"], nmax, C3["cd"], nmax=nmax, partner="melt", nk_chains=nk)
+    E = O.Ensemble(P, seed=args.seed, n=n, fp64=True, nk_per_chain=nk)
+    t0 = time.perf_counter()
+    ev = E.advance(t_end=T, threads=cpu_threads())[0]
+    sec = time.perf_counter() - t0
+    st = E.state()
+    ref = (st["Z"], st["N"], st["Q"])
+    out = {"workload": "first %d chains of the headline lengths (own melt partners), beta=1, CD on, %g tau_K"
+                       % (n, T)}
+    out.update(D.two_sample(dev, ref))
+    out["device_vs_equilibrium"] = D.summarize(*dev, nk, C3["beta"])
+    out["fp64_oracle_vs_equilibrium"] = D.summarize(*ref, nk, C3["beta"])
+    out["fp64_oracle_jumps"] = int(ev)
+    out["fp64_oracle_seconds"] = sec
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -314,7 +348,7 @@ def main():
     pvb = None if args.no_plain else run_plain_vs_batched(g, args)
     copy_bytes = 1 << 30
     copy_ms = g.probe_copy(copy_bytes, 20)
-    cpu = gte = None
+    cpu = gte = dse = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(args, c3["nk"])
@@ -324,6 +358,10 @@ def main():
             gte = gt_error(g, args)
         except Exception as exc:
             gte = {"failed": str(exc)}
+        try:
+            dse = dist_error(g, args, c3["nk"])
+        except Exception as exc:
+            dse = {"failed": str(exc)}
     out = {
         "metric": "KMC jumps/sec & G(t) err vs FP64 oracle, 1 MI355X, batched vs plain",
         "value": value,
@@ -348,6 +386,7 @@ def main():
         "roofline": rl,
         "cpu_baseline": cpu,
         "gt_err_vs_fp64": gte,
+        "dist_err_vs_fp64": dse,
         "jumps": c3["jumps"],
         "mean_Z_per_jump": c3["z_sum"] / max(c3["jumps"], 1),
         "kmc_device_ms": c3["kmc_ms"],

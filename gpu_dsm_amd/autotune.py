@@ -10,7 +10,9 @@ common/ensemble_kernel.cu:41-42).  Here the per-pass layout is chosen per worklo
   split_margin      how close to a wave-form geometry's capacity a chain may start in it;
   lds_table_floats  rate tables staged in LDS (one copy per wave) or read from global memory;
   blocks_per_cu     persistent waves per CU (occupancy below the register limit);
-  window            samples per staging window: the stress / correlator sync interval.
+  window            samples per staging window: the stress / correlator sync interval;
+  wide_pair         chains with 128 <= Z < 224 two per wave (two register groups each)
+                    instead of one per wave, in concurrent passes.
 
 No option changes a result (every chain makes the same jumps with the same Philox
 numbers under any setting; tests/test_gpu_options.py), so tuning is free to pick on
@@ -39,9 +41,10 @@ CANDIDATES = {
     "lds_table_floats": [2816, 0, 8192],
     "blocks_per_cu": [0, 1, 2],
     "window": [256, 16, 64],
+    "wide_pair": [0, 1],
 }
 DEFAULTS = {"small_per_cu": 12, "concurrent": -1, "split_margin": 8, "lds_table_floats": 2816, "blocks_per_cu": 0,
-            "window": 256}
+            "window": 256, "wide_pair": 0}
 
 
 def signature(descs):

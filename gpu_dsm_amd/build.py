@@ -26,7 +26,8 @@ COMMON = [
 ]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["dsm_kernels.hip", "dsm_engine.cpp", "dsm_driver.cpp", "dsm_serve.cpp", "dsm_noalloc.hip"]
-GEOMS = [0, 1, 2, 3, 4, 5, 6]  # KMC chain geometries (one translation unit each; 6 = row and pair merged)
+GEOMS = [0, 1, 2, 3, 4, 5, 6, 7]  # KMC chain geometries (one translation unit each; 6 = row and pair merged,
+                                  # 7 = pair form with two register groups)
 
 
 def hipcc():

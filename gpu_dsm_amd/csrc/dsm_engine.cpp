@@ -333,6 +333,7 @@ struct EngineOptions {
   int blocks_per_cu = 0;                 // DSM_OPT_BLOCKS_PER_CU
   int window = kDefaultWindow;           // DSM_OPT_WINDOW
   int graph = 0;                         // DSM_OPT_GRAPH
+  int wide_pair = 0;                     // DSM_OPT_WIDE_PAIR
 };
 
 struct dsm_engine {
@@ -764,11 +765,16 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // free first takes the remaining chains of those bins (a launch that finds its queue
   // empty exits at once).
   auto stream = [&](int k) { return k == 0 ? e->st : e->xs[k]; };  // xs[0] is not a stream
+  // bin 2 (128 <= Z < 224 at the split) runs two chains per wave in the two-group pair
+  // form when DSM_OPT_WIDE_PAIR is set (a third fewer VALU instructions per jump, the
+  // same window time on the 262,144-chain melt: 129.1 against 129.8 ms); the late round
+  // keeps one chain per wave
   auto first = [&](int b, int k) -> hipError_t {
     if (b > ps.bin_max) return hipSuccess;
     bind(kSplitList + b, late_of(b));
     ++rs->kmc_launches;
-    return launch_kmc(a, b, false, grid(b), stream(k));
+    const int geom = (b == 2 && e->opt.wide_pair) ? kGeomPair2 : b;
+    return launch_kmc(a, geom, false, grid(geom), stream(k));
   };
   HIPCHK(first(5, 0));
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
@@ -1708,6 +1714,7 @@ int dsm_set_option(dsm_engine* e, int32_t key, int32_t value) {
     case DSM_OPT_BLOCKS_PER_CU: if (value < 0) break; o.blocks_per_cu = value; return DSM_OK;
     case DSM_OPT_WINDOW: if (value < 1) break; o.window = value; return DSM_OK;
     case DSM_OPT_GRAPH: if (value < 0 || value > 1) break; o.graph = value; return DSM_OK;
+    case DSM_OPT_WIDE_PAIR: if (value < 0 || value > 1) break; o.wide_pair = value; return DSM_OK;
     default: return fail(DSM_ERR_ARG, "unknown option %d", key);
   }
   return fail(DSM_ERR_ARG, "option %d: value %d out of range", key, value);
@@ -1725,6 +1732,7 @@ int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value) {
     case DSM_OPT_BLOCKS_PER_CU: *value = o.blocks_per_cu; return DSM_OK;
     case DSM_OPT_WINDOW: *value = o.window; return DSM_OK;
     case DSM_OPT_GRAPH: *value = o.graph; return DSM_OK;
+    case DSM_OPT_WIDE_PAIR: *value = o.wide_pair; return DSM_OK;
     default: return fail(DSM_ERR_ARG, "unknown option %d", key);
   }
 }
@@ -1862,14 +1870,15 @@ static hipError_t l_philox(const void* i, void* o, int n, int, hipStream_t s) {
 }
 
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form) {
-  if (!in || !out || form < 0 || form > 2 || n <= 0 || n > (form == 1 ? 1024 : form == 2 ? 128 : 64))
+  if (!in || !out || form < 0 || form > 3 || n <= 0 ||
+      n > (form == 1 ? 1024 : form == 2 ? 128 : form == 3 ? 256 : 64))
     return fail(DSM_ERR_ARG, "bad probe arguments");
-  const int chains = form == 1 ? 1 : form == 2 ? 2 : 4;
+  const int chains = form == 1 ? 1 : (form == 2 || form == 3) ? 2 : 4;
   const int nin = chains * n, nout = chains * (n + 1);
   return probe_run(device_id, in, 4 * (size_t)nin, out, 4 * (size_t)nout, l_scan, n, form);
 }
 int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom) {
-  if (!in || !out || geom < 0 || geom > 4 || n < 1 || n > kmc_geom_capacity(geom))
+  if (!in || !out || geom < 0 || (geom > 4 && geom != kGeomPair2) || n < 1 || n > kmc_geom_capacity(geom))
     return fail(DSM_ERR_ARG, "bad probe arguments");
   return probe_run(device_id, in, 4 * (size_t)(4 * n + 1), out, 4 * 3, l_com, n, geom);
 }

@@ -644,10 +644,11 @@ __global__ void copy_probe_kernel(const float4* __restrict__ in, float4* __restr
 // Canonical scan tree of the KMC kernels.  form 1: one chain of n <= 1024 positions
 // (wave form, four register groups): out[0..n) inclusive sums, out[n] total.
 // form 0: four chains of n <= 64 positions (row form), form 2: two chains of n <= 128
-// positions (pair form): chain r reads in[r*n ..] and writes out[r*(n+1) ..].
+// positions (pair form), form 3: two chains of n <= 256 positions (pair form with two
+// register groups): chain r reads in[r*n ..] and writes out[r*(n+1) ..].
 // Centre-of-mass sample of one chain state in a chosen geometry (0 row form, 1 pair form,
-// 2/3/4 wave form G = 1/2/4): in[4i..4i+3] = Qx Qy Qz N of strand i, in[4n] = Nk;
-// out = 3 floats.
+// 2/3/4 wave form G = 1/2/4, 7 pair form with two groups): in[4i..4i+3] = Qx Qy Qz N of
+// strand i, in[4n] = Nk; out = 3 floats.
 template <int CPW, int G>
 __device__ void com_probe(const float* in, float* out, int n) {
   const LaneC<CPW> L;
@@ -672,6 +673,7 @@ __global__ void __launch_bounds__(64) com_probe_kernel(const float* in, float* o
   else if (geom == 1) com_probe<2, 1>(in, out, n);
   else if (geom == 2) com_probe<1, 1>(in, out, n);
   else if (geom == 3) com_probe<1, 2>(in, out, n);
+  else if (geom == kGeomPair2) com_probe<2, 2>(in, out, n);
   else com_probe<1, 4>(in, out, n);
 }
 
@@ -703,6 +705,22 @@ __global__ void __launch_bounds__(64) scan_probe_kernel(const float* in, float* 
     const float tot = chain_scan<2, 1>(v, g, n, L);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
+      const int p = L.pos(j);
+      if (p < n) out[r * (n + 1) + p] = g[j];
+    }
+    if (L.sl == 0) out[r * (n + 1) + n] = tot;
+  } else if (form == 3) {
+    const LaneC<2> L;
+    const int r = L.lane >> 5;
+    float v[8], g[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = L.pos(j);
+      v[j] = p < n ? in[r * n + p] : 0.0f;
+    }
+    const float tot = chain_scan<2, 2>(v, g, n, L);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
       const int p = L.pos(j);
       if (p < n) out[r * (n + 1) + p] = g[j];
     }
@@ -762,6 +780,7 @@ hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStrea
     case 3: return launch_kmc_geom_3(a, plain, grid, st);
     case 4: return launch_kmc_geom_4(a, plain, grid, st);
     case 5: return launch_kmc_geom_5(a, plain, grid, st);
+    case kGeomPair2: return launch_kmc_geom_7(a, plain, grid, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -774,6 +793,7 @@ int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext) {
     case 3: return kmc_occupancy_geom_3(plain, lds_floats, ext);
     case 4: return kmc_occupancy_geom_4(plain, lds_floats, ext);
     case 5: return kmc_occupancy_geom_5(plain, lds_floats, ext);
+    case kGeomPair2: return kmc_occupancy_geom_7(plain, lds_floats, ext);
     default: return 0;
   }
 }

@@ -120,11 +120,13 @@ __device__ __forceinline__ int split_newn(float r, int N, float wc) {
 // ------------------------------------------------------------------ geometry
 template <int CPW, int G>
 struct Geo {
-  static_assert(((CPW == 4 || CPW == 2) && G == 1) || (CPW == 1 && (G == 1 || G == 2 || G == 4 || G == 8)),
+  static_assert((CPW == 4 && G == 1) || (CPW == 2 && (G == 1 || G == 2)) ||
+                    (CPW == 1 && (G == 1 || G == 2 || G == 4 || G == 8)),
                 "unsupported chain geometry");
   static constexpr int R = 4 * G;                        // registers per strand array
   static constexpr int SEG = 64 / CPW;                   // lanes per chain
-  static constexpr int CAP = 4 * SEG * G;                // positions per chain
+  static constexpr int GS = 4 * SEG;                     // positions per register group of a chain
+  static constexpr int CAP = GS * G;                     // positions per chain
   static constexpr int STG = 256 * G;                    // floats per staging array (all chains of a wave)
 };
 
@@ -145,7 +147,8 @@ struct LaneC {
     pos0 = 4 * sl;
     srow = 4 * sb;
   }
-  __device__ __forceinline__ int pos(int j) const { return 256 * (j >> 2) + pos0 + (j & 3); }
+  // register group g = j/4 holds positions 4*SEG*g .. (the chain's segment, four slots per lane)
+  __device__ __forceinline__ int pos(int j) const { return (256 / CPW) * (j >> 2) + pos0 + (j & 3); }
 };
 
 // ------------------------------------------------------------------ cross-lane steps
@@ -218,6 +221,42 @@ __device__ __forceinline__ float wave_next_x(float cur, float nextg, int lane) {
   return __int_as_float(wave_next_x(__float_as_int(cur), __float_as_int(nextg), lane));
 }
 
+// Across a register-group boundary inside a chain's segment (wave form, or the pair form
+// with two groups): cross_prev gives segment lane sl > 0 `cur` of lane sl-1 and lane 0
+// `prevg` of the segment's last lane; cross_next gives lane sl < SEG-1 `cur` of lane
+// sl+1 and the segment's last lane `nextg` of its first lane.  The source register is
+// chosen per lane before the permute, as in wave_prev_x / wave_next_x.
+template <int CPW, typename T>
+__device__ __forceinline__ T cross_prev(T cur, T prevg, const LaneC<CPW>& L) {
+  if constexpr (CPW == 1) {
+    return wave_prev_x(cur, prevg, L.lane);
+  } else {
+    constexpr int SEG = 64 / CPW;
+    const T t = (L.sl == SEG - 1) ? prevg : cur;
+    int b;
+    __builtin_memcpy(&b, &t, 4);
+    b = __builtin_amdgcn_ds_bpermute((L.sb + ((L.sl + SEG - 1) & (SEG - 1))) << 2, b);
+    T r;
+    __builtin_memcpy(&r, &b, 4);
+    return r;
+  }
+}
+template <int CPW, typename T>
+__device__ __forceinline__ T cross_next(T cur, T nextg, const LaneC<CPW>& L) {
+  if constexpr (CPW == 1) {
+    return wave_next_x(cur, nextg, L.lane);
+  } else {
+    constexpr int SEG = 64 / CPW;
+    const T t = (L.sl == 0) ? nextg : cur;
+    int b;
+    __builtin_memcpy(&b, &t, 4);
+    b = __builtin_amdgcn_ds_bpermute((L.sb + ((L.sl + 1) & (SEG - 1))) << 2, b);
+    T r;
+    __builtin_memcpy(&r, &b, 4);
+    return r;
+  }
+}
+
 // value of v on segment lane `src` (per lane; 0 <= src < SEG)
 template <int CPW>
 __device__ __forceinline__ float seg_get(float v, int src, const LaneC<CPW>& L) {
@@ -257,6 +296,28 @@ __device__ __forceinline__ float chain_scan(const float (&v)[4 * G], float (&out
   if constexpr (CPW == 4) {
     block_scan4(v[0], v[1], v[2], v[3], out);
     return row_last(out[3]);
+  } else if constexpr (CPW == 2 && G >= 2) {
+    // chain s holds block 2g in row 2s and block 2g+1 in row 2s+1 of register group g;
+    // the carry runs over the blocks in position order, as in the wave form
+    const bool s1 = L.lane >= 32, upper = (L.lane & 16) != 0;
+    const int nb = ((Z - 1) >> 6) + 1;
+    float carry = 0.0f, total = 0.0f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      float I[4];
+      block_scan4(v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3], I);
+      const float b00 = rdl(I[3], 15), b01 = rdl(I[3], 31), b10 = rdl(I[3], 47), b11 = rdl(I[3], 63);
+      const float B0 = s1 ? b10 : b00, B1 = s1 ? b11 : b01;
+      const float c0 = carry;
+      carry = (g == 0) ? B0 : carry + B0;
+      total = (2 * g == nb - 1) ? carry : total;
+      const float c1 = carry;
+      carry = carry + B1;
+      total = (2 * g + 1 == nb - 1) ? carry : total;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out[4 * g + k] = upper ? c1 + I[k] : (g == 0 ? I[k] : c0 + I[k]);
+    }
+    return total;
   } else if constexpr (CPW == 2) {
     // blocks 0 and 1 of chain s are rows 2s and 2s+1; block 1 adds block 0's total
     float I[4];
@@ -313,6 +374,25 @@ __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr,
     cand = min(cand, dppi<kRowShr4>(cand, kIdent));
     cand = min(cand, dppi<kRowShr8>(cand, kIdent));
     return row_last(cand);
+  } else if constexpr (CPW == 2 && G >= 2) {
+    // per group: each lane's first slot, the row minimum, then the chain's two rows;
+    // later groups hold later positions, so the minimum over groups is the first one
+    constexpr int kIdent = 0x7fffffff;
+    int res = CAP;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int b0 = Geo<CPW, G>::GS * g + L.pos0;
+      int cand = CAP;
+#pragma unroll
+      for (int k = 3; k >= 0; --k) cand = (Gv[4 * g + k] > pr && b0 + k < Z) ? b0 + k : cand;
+      cand = min(cand, dppi<kRowShr1>(cand, kIdent));
+      cand = min(cand, dppi<kRowShr2>(cand, kIdent));
+      cand = min(cand, dppi<kRowShr4>(cand, kIdent));
+      cand = min(cand, dppi<kRowShr8>(cand, kIdent));
+      const int r0 = rdl(cand, 15), r1 = rdl(cand, 31), r2 = rdl(cand, 47), r3 = rdl(cand, 63);
+      res = min(res, L.lane >= 32 ? min(r2, r3) : min(r0, r1));
+    }
+    return res;
   } else if constexpr (CPW == 2) {
     int cand = CAP;
 #pragma unroll
@@ -346,7 +426,7 @@ __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr,
 // never indexed at run time (which would force it out of registers).
 template <int CPW, int G, typename T>
 __device__ __forceinline__ T value_at(const T (&a)[4 * G], int p, const LaneC<CPW>& L) {
-  const int j = 4 * (p >> 8) + (p & 3);
+  const int j = 4 * (p / Geo<CPW, G>::GS) + (p & 3);
   const int src = (p >> 2) & (Geo<CPW, G>::SEG - 1);
   T v = seg_get<CPW>(a[0], src, L);
 #pragma unroll
@@ -363,7 +443,7 @@ __device__ __forceinline__ void gap_open(T (&a)[4 * G], int at, int Z, const Lan
   T prev3[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    prev3[g] = (CPW == 1 && g > 0) ? wave_prev_x(a[4 * g + 3], a[g > 0 ? 4 * g - 1 : 0], L.lane)
+    prev3[g] = (g > 0) ? cross_prev<CPW, T>(a[4 * g + 3], a[g > 0 ? 4 * g - 1 : 0], L)
                                    : seg_prev<CPW, T>(a[4 * g + 3]);
   }
 #pragma unroll
@@ -379,7 +459,7 @@ __device__ __forceinline__ void gap_close(T (&a)[4 * G], int at, int Z, const La
   T next0[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    next0[g] = (CPW == 1 && g + 1 < G) ? wave_next_x(a[4 * g], a[4 * g + 4 < 4 * G ? 4 * g + 4 : 0], L.lane)
+    next0[g] = (g + 1 < G) ? cross_next<CPW, T>(a[4 * g], a[4 * g + 4 < 4 * G ? 4 * g + 4 : 0], L)
                                        : seg_next<CPW, T>(a[4 * g]);
   }
 #pragma unroll
@@ -395,7 +475,7 @@ template <int CPW, int G>
 __device__ __forceinline__ void shift_prev(const float (&a)[4 * G], float (&out)[4 * G], const LaneC<CPW>& L) {
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    out[4 * g] = (CPW == 1 && g > 0) ? wave_prev_x(a[4 * g + 3], a[g > 0 ? 4 * g - 1 : 0], L.lane)
+    out[4 * g] = (g > 0) ? cross_prev<CPW, float>(a[4 * g + 3], a[g > 0 ? 4 * g - 1 : 0], L)
                                      : seg_prev<CPW, float>(a[4 * g + 3]);
 #pragma unroll
     for (int k = 1; k < 4; ++k) out[4 * g + k] = a[4 * g + k - 1];
@@ -414,14 +494,14 @@ struct Stage {
   // this lane's four slots of register group g
   __device__ __forceinline__ void put4(int a, int g, int pos0, float v0, float v1, float v2, float v3) const {
     const vf4 v = {v0, v1, v2, v3};
-    *(lds_f4*)(arr(a) + 256 * g + pos0) = v;
+    *(lds_f4*)(arr(a) + Geo<CPW, G>::GS * g + pos0) = v;
   }
   __device__ __forceinline__ void put4i(int a, int g, int pos0, int v0, int v1, int v2, int v3) const {
     const vi4 v = {v0, v1, v2, v3};
-    *(lds_i4*)((lds_i*)arr(a) + 256 * g + pos0) = v;
+    *(lds_i4*)((lds_i*)arr(a) + Geo<CPW, G>::GS * g + pos0) = v;
   }
   __device__ __forceinline__ vf4 get4(int a, int g, int pos0) const {
-    return *(lds_f4*)(arr(a) + 256 * g + pos0);
+    return *(lds_f4*)(arr(a) + Geo<CPW, G>::GS * g + pos0);
   }
 };
 
@@ -519,9 +599,9 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int gn = g + 1 < G ? g + 1 : g;
-    const bool cross = CPW == 1 && g + 1 < G;
-    nn[g] = cross ? wave_next_x(C.n[4 * g], C.n[4 * gn], L.lane) : seg_next<CPW, int>(C.n[4 * g]);
-    qn[g] = cross ? wave_next_x(C.q2[4 * g], C.q2[4 * gn], L.lane) : seg_next<CPW, float>(C.q2[4 * g]);
+    const bool cross = g + 1 < G;
+    nn[g] = cross ? cross_next<CPW, int>(C.n[4 * g], C.n[4 * gn], L) : seg_next<CPW, int>(C.n[4 * g]);
+    qn[g] = cross ? cross_next<CPW, float>(C.q2[4 * g], C.q2[4 * gn], L) : seg_next<CPW, float>(C.q2[4 * g]);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -1046,7 +1126,7 @@ __device__ __forceinline__ void load_strands(Strands<G>& C, const ChainArrays& c
                                              const LaneC<CPW>& L) {
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int p0 = 256 * g + L.pos0;
+    const int p0 = Geo<CPW, G>::GS * g + L.pos0;
     float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x, z = x, w = x;
     int4 n = make_int4(0, 0, 0, 0);
     if (!H.inert && p0 < H.Z) {
@@ -1075,7 +1155,7 @@ __device__ __forceinline__ void store_strands(const Strands<G>& C, const ChainAr
   hi = min(hi, H.Z - 1);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int p0 = 256 * g + L.pos0;
+    const int p0 = Geo<CPW, G>::GS * g + L.pos0;
     const int64_t o = base + p0;
     if (p0 >= lo && p0 + 3 <= hi) {
       *(float4*)(ch.Qx + o) = make_float4(C.qx[4 * g], C.qx[4 * g + 1], C.qx[4 * g + 2], C.qx[4 * g + 3]);
@@ -1148,7 +1228,7 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
   if (H.inert) H.Z = 1;
   H.t_next = next_time(E, H.next_sample);
   H.e1 = 1.0f / (E.beta * (float)H.nk);
-  Stage<CPW, G> S{stg, L.srow};
+  Stage<CPW, G> S{stg, L.srow * G};
   Strands<G> C;
   load_strands<CPW, G>(C, a.ch, H, H.c >= 0 ? a.ch.base[H.c] : 0, L);
   stage_n<CPW, G>(S, C, L);
@@ -1221,7 +1301,7 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
   H.c = -1; H.Z = 1; H.nk = 1; H.stall = 0; H.local = 0; H.cid = 0; H.next_sample = 0; H.t = 0.0;
   H.t_next = 0.0; H.ev = 0; H.zsum = 0; H.dt_prev = 0.0f; H.left = 0; H.inert = 1; H.e1 = 0.0f;
   H.ovf = 0; H.done = 0; H.dlo = 1 << 30; H.dhi = -1;
-  Stage<CPW, G> S{stg, L.srow};
+  Stage<CPW, G> S{stg, L.srow * G};
   Strands<G> C;
 #pragma unroll
   for (int j = 0; j < 4 * G; ++j) { C.qx[j] = 0.f; C.qy[j] = 0.f; C.qz[j] = 0.f; C.q2[j] = 0.f; C.w[j] = 0.f; C.n[j] = 0; }

@@ -5,6 +5,8 @@
 //   2, 3, 4, 5: wave form, one chain per wave, Z <= 256, 512, 1024, 2048 (the last keeps
 //   part of its state in scratch memory; it carries the rare chains with Z > 1024)
 //   6: the row and pair forms in one launch (batched, concurrent passes)
+//   7: pair form with two register groups: two chains per wave, Z <= 256 (the bulk of
+//      128 <= Z < 224 in concurrent passes)
 #include "dsm_kmc.h"
 #include "dsm_launch.h"
 
@@ -53,9 +55,9 @@ int kmc_rowpair_occupancy(int lds_floats, bool ext) {
 #else
 
 namespace {
-constexpr int kCPW = DSM_GEOM == 0 ? 4 : (DSM_GEOM == 1 ? 2 : 1);
-constexpr int kG = DSM_GEOM <= 2 ? 1 : (DSM_GEOM == 3 ? 2 : (DSM_GEOM == 4 ? 4 : 8));
-static_assert(Geo<kCPW, kG>::CAP == (64 << DSM_GEOM), "geometry g holds 64 << g positions");
+constexpr int kCPW = DSM_GEOM == 0 ? 4 : ((DSM_GEOM == 1 || DSM_GEOM == 7) ? 2 : 1);
+constexpr int kG = DSM_GEOM <= 2 ? 1 : ((DSM_GEOM == 3 || DSM_GEOM == 7) ? 2 : (DSM_GEOM == 4 ? 4 : 8));
+static_assert(Geo<kCPW, kG>::CAP == kmc_geom_capacity(DSM_GEOM), "positions per chain of the geometry");
 // LDS a workgroup may hold: the rate tables are read from global memory when the staging
 // arrays and the table copy would not fit together
 constexpr int kMaxLdsFloats = 65536 / 4;

@@ -16,7 +16,10 @@ namespace dsm {
 constexpr int kNumGeoms = 6;
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st);
 int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext);
-__host__ __device__ inline int kmc_geom_capacity(int geom) { return 64 << geom; }  // positions per chain
+// Kernel geometries 0..5 hold 64 << g positions per chain (the work bins of a pass);
+// geometry 7 is the two-group pair form, a second kernel for bin 2 (256 positions).
+constexpr int kGeomPair2 = 7;
+__host__ __device__ constexpr int kmc_geom_capacity(int geom) { return geom == kGeomPair2 ? 256 : 64 << geom; }
 #define DSM_DECLARE_GEOM(G)                                                                   \
   hipError_t launch_kmc_geom_##G(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
   int kmc_occupancy_geom_##G(bool plain, int lds_floats, bool ext);
@@ -26,6 +29,7 @@ DSM_DECLARE_GEOM(2)
 DSM_DECLARE_GEOM(3)
 DSM_DECLARE_GEOM(4)
 DSM_DECLARE_GEOM(5)
+DSM_DECLARE_GEOM(7)
 #undef DSM_DECLARE_GEOM
 // the row form (list `row`) and the pair form (list `pair`) in one batched launch
 hipError_t launch_kmc_rowpair(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st);

@@ -27,7 +27,7 @@ EXPORTS = ["dsm_create", "dsm_destroy", "dsm_last_error", "dsm_mem_info", "dsm_a
            "dsm_ensemble_info", "dsm_save", "dsm_load", "dsm_load_error", "dsm_stress", "dsm_check_state",
            "dsm_serve_ex", "dsm_correlator_range", "dsm_set_option", "dsm_get_option", "dsm_stress_noalloc"]
 OPTIONS = {"plain_jumps": 1, "small_per_cu": 2, "split_margin": 3, "lds_table_floats": 4, "concurrent": 5,
-           "blocks_per_cu": 6, "window": 7, "graph": 8}
+           "blocks_per_cu": 6, "window": 7, "graph": 8, "wide_pair": 9}
 CHECKS = ["z_range", "n_below_1", "n_sum", "end_q", "nonfinite", "stalled", "behind"]
 
 
@@ -397,7 +397,7 @@ class Engine:
 def probe_com(Q, N, geom, device=0):
     """MSD centre-of-mass sample (relative to R1) of one chain state computed by the
     KMC kernels' code in geometry `geom` (0 row form, 1 pair form, 2/3/4 wave form
-    G = 1/2/4)."""
+    G = 1/2/4, 7 pair form with two register groups)."""
     Q = np.asarray(Q, np.float32).reshape(-1, 3)
     N = np.asarray(N)
     n = Q.shape[0]
@@ -412,14 +412,15 @@ def probe_com(Q, N, geom, device=0):
 def probe_scan(values, form=1, device=0):
     """Canonical scan tree on the device.  form 1: one chain (len <= 1024) -> (inclusive
     sums, total).  form 0: values shaped (4, n<=64), four row-form chains -> (sums (4, n),
-    totals (4,)); form 2: values shaped (2, n<=128), two pair-form chains."""
+    totals (4,)); form 2: values shaped (2, n<=128), two pair-form chains; form 3: (2, n<=256),
+    two chains of the pair form with two register groups."""
     v = np.ascontiguousarray(values, np.float32)
     if form == 1:
         n = v.size
         out = np.zeros(n + 1, np.float32)
         _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, 1))
         return out[:n], out[n]
-    rows = 2 if form == 2 else 4
+    rows = 2 if form in (2, 3) else 4
     v = v.reshape(rows, -1)
     n = v.shape[1]
     out = np.zeros((rows, n + 1), np.float32)

@@ -13,11 +13,13 @@ bench.py at the repo root measures the headline (the c3 melt, 16 tau_K windows) 
 import argparse
 import json
 import math
+import os
 import sys
 import time
 
 import numpy as np
 
+from . import distributions as dist
 from . import engine as eng
 
 
@@ -54,10 +56,14 @@ def c3_lengths(n=262144, seed=1, nk_lo=10, nk_hi=2000):
     return np.clip(nk, nk_lo, nk_hi).astype(np.int32)
 
 
-def c3(horizon=100.0, seed=1, n=262144):
+def c3(horizon=100.0, seed=1, n=262144, distr=None):
     """Polydisperse melt: 262,144 chains of log-normal length (Mn 200, PDI 1.5), beta=1,
     CD on with constraint-release partners drawn from the melt's own length distribution
-    (weight-averaged), G(t) with p=16, m=2, all chains in one batched pass per window."""
+    (weight-averaged), G(t) with p=16, m=2, all chains in one batched pass per window.
+    The output carries the Z / N / Q distributions of the final state against the
+    equilibrium laws; `distr` (a directory) also receives them as the reference's
+    distr_Z.dat, distr_N.dat and distr_Q.dat tables (the |Q| CDF at 100,000 evenly spaced
+    rows instead of one row per internal strand: 2.6e7 here)."""
     nk = c3_lengths(n, seed)
     p, m = 16, 2
     s = int(math.ceil(math.log(horizon / p) / math.log(m))) + 1
@@ -69,14 +75,23 @@ def c3(horizon=100.0, seed=1, n=262144):
         t0 = time.perf_counter()
         st = e.advance(ens, horizon)
         wall = time.perf_counter() - t0
-        z1 = e.state(ens)["Z"]
         lag, g = e.correlator(ens)
+        Z1, N1, Q1 = dist.fetch_zn_q(e, ens)
+    z1 = Z1
+    dsum = dist.summarize(Z1, N1, Q1, nk, 1.0)
+    if distr:
+        os.makedirs(distr, exist_ok=True)
+        tables = dist.reference_files(Z1, N1, Q1, max_q_rows=100000)
+        for name, (x, y) in zip(("distr_Z.dat", "distr_N.dat", "distr_Q.dat"), tables):
+            with open(os.path.join(distr, name), "w") as f:
+                f.writelines("%s\t%.9g\n" % (a, b) for a, b in zip(x.tolist(), y.tolist()))
+    del N1, Q1
     return dict(workload="c3: %d chains, Nk log-normal Mn 200 PDI 1.5 clipped to [10, %d], beta=1, CD on "
                          "(melt partners), G(t)"
                 % (n, int(nk.max())), horizon_tau_k=horizon, jumps=st["events"], seconds=wall,
                 jumps_per_s=st["events"] / wall, kmc_ms=st["kmc_ms"], first_kmc_ms=st["first_kmc_ms"],
                 mean_Z_start=float(z0.mean()), mean_Z_end=float(z1.mean()), max_Z=int(z1.max()),
-                chains_Z_over_64=int((z1 > 64).sum()), G0=float(g[0]))
+                chains_Z_over_64=int((z1 > 64).sum()), G0=float(g[0]), distributions=dsum)
 
 
 def c4(horizon=1.0e5, seed=1, segment=5000.0):
@@ -176,6 +191,7 @@ def main():
     ap.add_argument("--requests", type=int, default=None, help="c5: number of requests")
     ap.add_argument("--window", type=int, default=None, help="c1: samples per staging window")
     ap.add_argument("--graph", type=int, default=None, help="c1, c5: 1 = windows as captured HIP graphs")
+    ap.add_argument("--distr", default=None, help="c3: directory for the distr_Z/N/Q.dat tables")
     args = ap.parse_args()
     fn = dict(c1=c1, c3=c3, c4=c4, c5=c5, eta0=eta0)[args.which]
     kw = dict(seed=args.seed)
@@ -187,6 +203,8 @@ def main():
         kw["window"] = args.window
     if args.graph is not None and args.which in ("c1", "c5"):
         kw["graph"] = args.graph
+    if args.distr is not None and args.which == "c3":
+        kw["distr"] = args.distr
     print(json.dumps(fn(**kw)), flush=True)
 
 

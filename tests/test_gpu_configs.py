@@ -184,3 +184,29 @@ def test_c5_full_stream_completes():
     assert rep["makespan_s"] >= plan["arrival_s"][-1]
     print("C5 full stream (compressed): p50 %.3f s p99 %.3f s max active %d, %.3g jumps/s" % (
         rep["p50_s"], rep["p99_s"], rep["max_active"], rep["jumps_per_s"]))
+
+
+@pytest.mark.parametrize("cd,partner", [(0, None), (1, None)], ids=["cd_off", "cd_own_background"])
+def test_c3_distributions_stay_at_equilibrium(engine, cd, partner):
+    """The Z / N / Q distributions (the reference's -distr outputs) of a run on the C3
+    lengths: 65,536 chains, 40 tau_K through the concurrent pass, against the
+    equilibrium laws (binomial Z, the internal-strand length law, chi-squared(3) for
+    3|Q|^2/N), in sampling-error units.  With CD off and with each chain's own CD
+    background the dynamics keeps these laws exactly.  (With the melt background the
+    internal-strand lengths drift from the law by up to 0.2%, the same in the FP64
+    restatement; that case is measured and reported, not asserted.)"""
+    from gpu_dsm_amd import distributions as D
+    from gpu_dsm_amd.workloads import c3_lengths
+    nk = c3_lengths(262144, seed=1)[:65536]
+    nmax = int(nk.max())
+    kw = dict(partner=partner) if partner else {}
+    ens = engine.add_ensemble(nk.size, nmax, 1.0, cd, seed=3, sample="none", nk_per_chain=nk, **kw)
+    engine.init_equilibrium(ens)
+    engine.advance(ens, 40.0)
+    Z, N, Q = D.fetch_zn_q(engine, ens)
+    engine.remove_ensemble(ens)
+    out = D.summarize(Z, N, Q, nk, 1.0)
+    print("C3 lengths, cd=%d: distributions after 40 tau_K:" % cd, out["Z"], out["N"]["max_abs_dev_sigma"], out["Q"])
+    assert abs(out["Z"]["mean_u_sigma"]) < 5 and abs(out["Z"]["var_u_sigma"]) < 5, out["Z"]
+    assert out["N"]["max_abs_dev_sigma"] < 5, out["N"]
+    assert out["Q"]["ks_sqrt_n"] < 2.5, out["Q"]

@@ -52,7 +52,7 @@ def test_philox_matches_rocrand_vectors():
 
 
 def test_scan_tree_matches_oracle(oracle):
-    """Both chain geometries of the KMC kernels sum in the oracle's canonical tree."""
+    """Every chain geometry of the KMC kernels sums in the oracle's canonical tree."""
     import gpu_dsm_amd as g
     rng = np.random.default_rng(5)
     for n in (1, 3, 17, 64, 65, 200, 256, 257, 640, 1000, 1024):
@@ -71,6 +71,13 @@ def test_scan_tree_matches_oracle(oracle):
     for n in (1, 7, 63, 64, 65, 100, 127, 128):
         vals = (rng.exponential(size=(2, n)) * 50).astype(np.float32)
         gpu, tot = g.probe_scan(vals, form=2)
+        for r in range(2):
+            assert np.array_equal(_bits(gpu[r]), _bits(oracle.scan_tree(vals[r])))
+            assert _bits(np.float32(tot[r])) == _bits(np.float32(oracle.tree_total(vals[r])))
+    # pair form with two register groups: block carries across rows and groups
+    for n in (1, 64, 65, 128, 129, 130, 191, 192, 193, 200, 255, 256):
+        vals = (rng.exponential(size=(2, n)) * 50).astype(np.float32)
+        gpu, tot = g.probe_scan(vals, form=3)
         for r in range(2):
             assert np.array_equal(_bits(gpu[r]), _bits(oracle.scan_tree(vals[r])))
             assert _bits(np.float32(tot[r])) == _bits(np.float32(oracle.tree_total(vals[r])))
@@ -230,7 +237,8 @@ def test_gt_samples_and_correlator(engine, oracle):
 
 @pytest.mark.parametrize("Z,geom", [(1, 0), (2, 0), (60, 0), (64, 0), (1, 1), (63, 1), (65, 1), (100, 1),
                                      (128, 1), (65, 2), (250, 2), (256, 2), (257, 3), (268, 3), (500, 3),
-                                     (512, 3), (513, 4), (900, 4), (1024, 4)])
+                                     (512, 3), (513, 4), (900, 4), (1024, 4), (1, 7), (100, 7), (128, 7),
+                                     (129, 7), (130, 7), (200, 7), (256, 7)])
 def test_com_sample_matches_oracle(oracle, Z, geom):
     """The centre-of-mass sample of the kernels, in every geometry, against the oracle."""
     import gpu_dsm_amd as g
