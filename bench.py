@@ -35,6 +35,10 @@ Printed: one JSON line.
   gt_err_vs_fp64  1000 chains, Nk 25, CD off, to 1e5 tau_K (the small configuration in full):
                device G(t) against the FP64 oracle, paired per-chain differences with
                standard errors, over the lags where G >= 1% of G(0).
+  stress_err_vs_fp64  shear from equilibrium of the sweep's chains (512 chains, Nk 50, CD on,
+               500 tau_K, rates 1e-3, 1e-2, 1e-1): means of sigma_xy and N1 over the second
+               half (strain 0.25-0.5, 2.5-5, 25-50: start-up, not yet steady at the lower
+               rates), device against the FP64 oracle on the same chains and Philox streams.
   dist_err_vs_fp64  the Z / N / Q distributions of the first 4096 chains of the headline
                ensemble (own melt partners) after 100 tau_K: device against the FP64
                oracle on the same chains and Philox streams (largest pmf / CDF
@@ -287,6 +291,34 @@ def gt_error(g, args):
             "fp64_oracle_seconds": sec64}
 
 
+def stress_error(g, args, n=512, nk=50, T=500.0):
+    """Shear stress of the same chains from equilibrium: device (FP32 flow path) vs FP64
+    oracle, second-half means."""
+    from oracle import oracle as O
+    out = {"workload": "%d chains, Nk=%d, beta=1, CD on, shear, %g tau_K, second-half means" % (n, nk, T),
+           "rates": []}
+    for rate in (1e-3, 1e-2, 1e-1):
+        kappa = [0, rate, 0, 0, 0, 0, 0, 0, 0]
+        with g.Engine(0) as eng:
+            ens = eng.add_ensemble(n, nk, 1.0, 1, kappa=kappa, seed=args.seed + 30, sample="flow", delta=1.0,
+                                   window=64)
+            eng.init_equilibrium(ens)
+            eng.advance(ens, T)
+            rows = eng.flow_stress(ens)
+        E = O.Ensemble(O.Params(1.0, nk, 1, kappa=kappa), seed=args.seed + 30, n=n, fp64=True)
+        _, smp = E.advance(t_end=T, delta=1.0, k0=1, k_end=int(T) + 1, samples=True, flow_samples=True,
+                           threads=cpu_threads())
+        ref = smp.sum(axis=1)
+        half = slice(rows.shape[0] // 2, None)
+        sxy, sxy64 = float(np.mean(rows[half, 3] / rows[half, 8])), float(np.mean(ref[half, 3] / n))
+        n1, n164 = (float(np.mean((rows[half, 0] - rows[half, 1]) / rows[half, 8])),
+                    float(np.mean((ref[half, 0] - ref[half, 1]) / n)))
+        out["rates"].append({"rate": rate, "sigma_xy": sxy, "sigma_xy_fp64": sxy64,
+                             "rel_err_sigma_xy": abs(sxy - sxy64) / abs(sxy64), "N1": n1, "N1_fp64": n164,
+                             "rel_err_N1": abs(n1 - n164) / abs(n164)})
+    return out
+
+
 def dist_error(g, args, nk_full, n=4096, T=100.0):
     """Z / N / Q distributions of the same chains after T tau_K: device vs FP64 oracle,
     and each against the equilibrium laws (gpu_dsm_amd.distributions)."""
@@ -348,7 +380,7 @@ def main():
     pvb = None if args.no_plain else run_plain_vs_batched(g, args)
     copy_bytes = 1 << 30
     copy_ms = g.probe_copy(copy_bytes, 20)
-    cpu = gte = dse = None
+    cpu = gte = dse = sse = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(args, c3["nk"])
@@ -362,6 +394,10 @@ def main():
             dse = dist_error(g, args, c3["nk"])
         except Exception as exc:
             dse = {"failed": str(exc)}
+        try:
+            sse = stress_error(g, args)
+        except Exception as exc:
+            sse = {"failed": str(exc)}
     out = {
         "metric": "KMC jumps/sec & G(t) err vs FP64 oracle, 1 MI355X, batched vs plain",
         "value": value,
@@ -387,6 +423,7 @@ def main():
         "cpu_baseline": cpu,
         "gt_err_vs_fp64": gte,
         "dist_err_vs_fp64": dse,
+        "stress_err_vs_fp64": sse,
         "jumps": c3["jumps"],
         "mean_Z_per_jump": c3["z_sum"] / max(c3["jumps"], 1),
         "kmc_device_ms": c3["kmc_ms"],
