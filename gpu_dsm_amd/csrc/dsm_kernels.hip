@@ -661,7 +661,6 @@ __device__ void com_probe(const float* in, float* out, int n) {
     C.qy[j] = on ? in[4 * p + 1] : 0.0f;
     C.qz[j] = on ? in[4 * p + 2] : 0.0f;
     C.n[j] = on ? (int)in[4 * p + 3] : 0;
-    C.q2[j] = 0.0f;
     C.w[j] = 0.0f;
   }
   float sv[3];

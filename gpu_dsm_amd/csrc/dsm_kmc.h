@@ -515,7 +515,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // ------------------------------------------------------------------ chain state
 template <int G>
 struct Strands {
-  float qx[4 * G], qy[4 * G], qz[4 * G], q2[4 * G], w[4 * G];
+  // (|Q|^2 of a strand is recomputed from qx, qy, qz where it is needed: the same FP32
+  // expression every time, so no register copy of it is kept)
+  float qx[4 * G], qy[4 * G], qz[4 * G], w[4 * G];
   int n[4 * G];
 };
 
@@ -580,20 +582,30 @@ __device__ __forceinline__ void stage_n(const Stage<CPW, G>& S, const Strands<G>
   for (int g = 0; g < G; ++g)
     S.put4i(kStN, g, L.pos0, C.n[4 * g], C.n[4 * g + 1], C.n[4 * g + 2], C.n[4 * g + 3]);
 }
+// Stage |Q|^2 and the CD rates; q2 receives the |Q|^2 values for a refresh_all that
+// follows.
 template <int CPW, int G>
-__device__ __forceinline__ void stage_qw(const Stage<CPW, G>& S, const Strands<G>& C, const LaneC<CPW>& L) {
+__device__ __forceinline__ void stage_qw(const Stage<CPW, G>& S, const Strands<G>& C, const LaneC<CPW>& L,
+                                         float (&q2)[4 * G]) {
+#pragma unroll
+  for (int j = 0; j < 4 * G; ++j) q2[j] = q2_of(C.qx[j], C.qy[j], C.qz[j]);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    S.put4(kStQ2, g, L.pos0, C.q2[4 * g], C.q2[4 * g + 1], C.q2[4 * g + 2], C.q2[4 * g + 3]);
+    S.put4(kStQ2, g, L.pos0, q2[4 * g], q2[4 * g + 1], q2[4 * g + 2], q2[4 * g + 3]);
     S.put4(kStW, g, L.pos0, C.w[4 * g], C.w[4 * g + 1], C.w[4 * g + 2], C.w[4 * g + 3]);
   }
+}
+template <int CPW, int G>
+__device__ __forceinline__ void stage_qw(const Stage<CPW, G>& S, const Strands<G>& C, const LaneC<CPW>& L) {
+  float q2[4 * G];
+  stage_qw<CPW, G>(S, C, L, q2);
 }
 
 // Recompute every cached pair rate of the chain (after a load or a deformation):
 // each lane evaluates the pairs of its own slots.
 template <int CPW, int G, bool LT>
-__device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strands<G>& C, const Tabs<LT>& T, int cd,
-                                            int Z, const LaneC<CPW>& L) {
+__device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strands<G>& C, const float (&q2)[4 * G],
+                                            const Tabs<LT>& T, int cd, int Z, const LaneC<CPW>& L) {
   int nn[G];
   float qn[G];
 #pragma unroll
@@ -601,7 +613,7 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
     const int gn = g + 1 < G ? g + 1 : g;
     const bool cross = g + 1 < G;
     nn[g] = cross ? cross_next<CPW, int>(C.n[4 * g], C.n[4 * gn], L) : seg_next<CPW, int>(C.n[4 * g]);
-    qn[g] = cross ? cross_next<CPW, float>(C.q2[4 * g], C.q2[4 * gn], L) : seg_next<CPW, float>(C.q2[4 * g]);
+    qn[g] = cross ? cross_next<CPW, float>(q2[4 * g], q2[4 * gn], L) : seg_next<CPW, float>(q2[4 * g]);
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -611,9 +623,9 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
       const int j = 4 * g + k;
       const int p = L.pos(j);
       const int Nj = (k < 3) ? C.n[j + (k < 3 ? 1 : 0)] : nn[g];
-      const float q2j = (k < 3) ? C.q2[j + (k < 3 ? 1 : 0)] : qn[g];
+      const float q2j = (k < 3) ? q2[j + (k < 3 ? 1 : 0)] : qn[g];
       wp[k] = 0.0f; wm[k] = 0.0f; rr[k] = 0.0f;
-      if (p <= Z - 2) pair_rate<LT>(T, cd, C.n[j], Nj, C.q2[j], q2j, C.w[j], p == 0, p + 1 == Z - 1, wp[k], wm[k], rr[k]);
+      if (p <= Z - 2) pair_rate<LT>(T, cd, C.n[j], Nj, q2[j], q2j, C.w[j], p == 0, p + 1 == Z - 1, wp[k], wm[k], rr[k]);
     }
     S.put4(kStWP, g, L.pos0, wp[0], wp[1], wp[2], wp[3]);
     S.put4(kStWM, g, L.pos0, wm[0], wm[1], wm[2], wm[3]);
@@ -696,7 +708,6 @@ __device__ __forceinline__ void deform(Strands<G>& C, const EnsParams* P, float 
       C.qx[j] = __builtin_fmaf(dk[2], z, __builtin_fmaf(dk[1], y, __builtin_fmaf(dk[0], x, x)));
       C.qy[j] = __builtin_fmaf(dk[5], z, __builtin_fmaf(dk[4], y, __builtin_fmaf(dk[3], x, y)));
       C.qz[j] = __builtin_fmaf(dk[8], z, __builtin_fmaf(dk[7], y, __builtin_fmaf(dk[6], x, z)));
-      C.q2[j] = q2_of(C.qx[j], C.qy[j], C.qz[j]);
     }
   }
 }
@@ -1079,22 +1090,21 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
     if (any_open) {
       const int ao = gap > 0 ? at : CAP;
       gap_open<CPW, G>(C.qx, ao, Z, L); gap_open<CPW, G>(C.qy, ao, Z, L); gap_open<CPW, G>(C.qz, ao, Z, L);
-      gap_open<CPW, G>(C.q2, ao, Z, L); gap_open<CPW, G>(C.w, ao, Z, L); gap_open<CPW, G>(C.n, ao, Z, L);
+      gap_open<CPW, G>(C.w, ao, Z, L); gap_open<CPW, G>(C.n, ao, Z, L);
       if (!EXT || !E.flow) gap_lds<CPW, G>(S, 1, ao, Z, L);
     }
     if (any_close) {
       const int ac = gap < 0 ? at : CAP;
       gap_close<CPW, G>(C.qx, ac, Z, L); gap_close<CPW, G>(C.qy, ac, Z, L); gap_close<CPW, G>(C.qz, ac, Z, L);
-      gap_close<CPW, G>(C.q2, ac, Z, L); gap_close<CPW, G>(C.w, ac, Z, L); gap_close<CPW, G>(C.n, ac, Z, L);
+      gap_close<CPW, G>(C.w, ac, Z, L); gap_close<CPW, G>(C.n, ac, Z, L);
       if (!EXT || !E.flow) gap_lds<CPW, G>(S, -1, ac, Z, L);
     }
     // patches
-    const float q0 = q2_of(p0x, p0y, p0z), q1 = q2_of(p1x, p1y, p1z);
 #pragma unroll
     for (int j = 0; j < 4 * G; ++j) {
       const int ps = L.pos(j);
-      if (ps == p0) { C.n[j] = p0N; C.qx[j] = p0x; C.qy[j] = p0y; C.qz[j] = p0z; C.q2[j] = q0; }
-      if (ps == p1) { C.n[j] = p1N; C.qx[j] = p1x; C.qy[j] = p1y; C.qz[j] = p1z; C.q2[j] = q1; }
+      if (ps == p0) { C.n[j] = p0N; C.qx[j] = p0x; C.qy[j] = p0y; C.qz[j] = p0z; }
+      if (ps == p1) { C.n[j] = p1N; C.qx[j] = p1x; C.qy[j] = p1y; C.qz[j] = p1z; }
       if (ps == w0i) C.w[j] = w0;
       if (ps == w1i) C.w[j] = w1;
     }
@@ -1143,7 +1153,6 @@ __device__ __forceinline__ void load_strands(Strands<G>& C, const ChainArrays& c
       const bool on = p0 + k < H.Z;
       C.qx[j] = on ? xs[k] : 0.0f; C.qy[j] = on ? ys[k] : 0.0f; C.qz[j] = on ? zs[k] : 0.0f;
       C.w[j] = on ? ws[k] : 0.0f; C.n[j] = on ? ns[k] : 0;
-      C.q2[j] = q2_of(C.qx[j], C.qy[j], C.qz[j]);
     }
   }
 }
@@ -1232,8 +1241,11 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
   Strands<G> C;
   load_strands<CPW, G>(C, a.ch, H, H.c >= 0 ? a.ch.base[H.c] : 0, L);
   stage_n<CPW, G>(S, C, L);
-  stage_qw<CPW, G>(S, C, L);
-  refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+  {
+    float q2[4 * G];
+    stage_qw<CPW, G>(S, C, L, q2);
+    refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
+  }
   wave_lds_sync();
   RandCache RC{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int k_rng = SEG;
@@ -1242,8 +1254,9 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
       const bool dfm = !H.inert && H.dt_prev != 0.0f;
       if (__ballot(dfm)) {
         if (dfm) deform<CPW, G>(C, E.P, H.dt_prev, H.Z, L);
-        stage_qw<CPW, G>(S, C, L);
-        refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+        float q2[4 * G];
+        stage_qw<CPW, G>(S, C, L, q2);
+        refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
         wave_lds_sync();
         if (dfm) { H.dlo = 0; H.dhi = max(H.dhi, H.Z - 1); }
       }
@@ -1304,7 +1317,7 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
   Stage<CPW, G> S{stg, L.srow * G};
   Strands<G> C;
 #pragma unroll
-  for (int j = 0; j < 4 * G; ++j) { C.qx[j] = 0.f; C.qy[j] = 0.f; C.qz[j] = 0.f; C.q2[j] = 0.f; C.w[j] = 0.f; C.n[j] = 0; }
+  for (int j = 0; j < 4 * G; ++j) { C.qx[j] = 0.f; C.qy[j] = 0.f; C.qz[j] = 0.f; C.w[j] = 0.f; C.n[j] = 0; }
   RandCache RC{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int k_rng = SEG;
   for (;;) {
@@ -1362,8 +1375,11 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
         load_strands<CPW, G>(C, a.ch, H, a.ch.base[c], L);
       }
       stage_n<CPW, G>(S, C, L);
-      stage_qw<CPW, G>(S, C, L);
-      refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+      {
+        float q2[4 * G];
+        stage_qw<CPW, G>(S, C, L, q2);
+        refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
+      }
       wave_lds_sync();
       // the cache is refilled at every chain's current jump count before the next jump
       // (clearing it here tells the compiler the old values are dead in this branch)
@@ -1377,8 +1393,9 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
         const bool dfm = !H.inert && H.dt_prev != 0.0f;
         if (__ballot(dfm)) {
           if (dfm) deform<CPW, G>(C, E.P, H.dt_prev, H.Z, L);
-          stage_qw<CPW, G>(S, C, L);
-          refresh_all<CPW, G, LT>(S, C, T, E.cd, H.Z, L);
+          float q2[4 * G];
+          stage_qw<CPW, G>(S, C, L, q2);
+          refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
           wave_lds_sync();
         }
         H.dt_prev = 0.0f;
