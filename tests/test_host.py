@@ -78,3 +78,33 @@ def test_oracle_is_not_imported_by_product():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 txt = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in txt.replace("the oracle", "").lower() or f == "build.py", f
+
+
+def test_c_program_compiles_links_and_runs(tmp_path):
+    """include/dsm.h is a C header: a C11 program using the engine calls compiles with
+    -Wall -Wextra -Werror, links against the in-tree library and runs (stopping at
+    dsm_create with its message when no device is present)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    exe = str(tmp_path / "c_abi_example")
+    lib = os.path.join(ROOT, "gpu_dsm_amd", "_build")
+    r = subprocess.run([cc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "c_abi_example.c"), "-L", lib, "-lgpu_dsm",
+                        "-Wl,-rpath," + lib, "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith(("dsm_create:", "jumps ")), r.stdout
+    return r.stdout
+
+
+@pytest.mark.gpu
+def test_c_program_runs_on_the_device(tmp_path):
+    """The same C program on a device: it advances an ensemble and reads stress and G(t)."""
+    out = test_c_program_compiles_links_and_runs(tmp_path)
+    assert out.startswith("jumps "), out
+    jumps = int(out.split()[1].rstrip(","))
+    assert jumps > 0
