@@ -1431,13 +1431,17 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 // Persistent kernel: each wave takes work units from the queue until it is empty.
 // PLAIN: a.plain_jumps jumps per chain, the state in HBM between launches (the
 // per-launch form; one jump per launch is the reference's structure); otherwise run to
-// the targets.  Both forms are held to the same occupancy targets.
+// the targets.  Both forms are held to the same occupancy targets: three waves per SIMD
+// (168 VGPRs) up to G = 2 (the G = 2 wave form spills 36 bytes per lane there, and still
+// finished the 262,144-chain melt's window in 117-126 ms against 130-131 ms at two waves:
+// its chains, each a serial run of about 2,000 jumps per window, need waves in flight),
+// one for G >= 4 (AGPRs as well).
 // EXT: the variant with the less common features, launched when an ensemble of the pass
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
 // centre-of-mass samples), the polydisperse CD background (partner tables) and flow
 // (no incremental rate refresh: the affine step recomputes every rate).
 template <int CPW, int G, bool LT, bool PLAIN, bool EXT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW == 1 && G != 2) ? 1 : (CPW == 1 ? 2 : 3)))) kmc_kernel(KmcArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW == 1 && G >= 4) ? 1 : 3))) kmc_kernel(KmcArgs a) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<CPW, G>();
