@@ -11,7 +11,7 @@ common/ensemble_kernel.cu:41-42).  Here the per-pass layout is chosen per worklo
   lds_table_floats  rate tables staged in LDS (one copy per wave) or read from global memory;
   blocks_per_cu     persistent waves per CU (occupancy below the register limit);
   window            samples per staging window: the stress / correlator sync interval;
-  wide_pair         chains with 128 <= Z < 224 two per wave (two register groups each)
+  wide_pair         chains with 128 <= Z < 240 two per wave (two register groups each)
                     instead of one per wave, in concurrent passes.
 
 No option changes a result (every chain makes the same jumps with the same Philox
@@ -37,13 +37,13 @@ from . import engine as eng
 CANDIDATES = {
     "small_per_cu": [12, 0, 4, 24, 48],
     "concurrent": [-1, 0, 1],
-    "split_margin": [8, 0, 4, 16],
+    "split_margin": [16, 0, 8, 32],
     "lds_table_floats": [2816, 0, 8192],
     "blocks_per_cu": [0, 1, 2],
     "window": [256, 16, 64],
     "wide_pair": [0, 1],
 }
-DEFAULTS = {"small_per_cu": 12, "concurrent": -1, "split_margin": 8, "lds_table_floats": 2816, "blocks_per_cu": 0,
+DEFAULTS = {"small_per_cu": 12, "concurrent": -1, "split_margin": 16, "lds_table_floats": 2816, "blocks_per_cu": 0,
             "window": 256, "wide_pair": 0}
 
 
@@ -103,6 +103,16 @@ def _rate(e, ids, t, dt, reps):
     return ev / wall, t
 
 
+def _settle(best, best_rate, base, base2, min_gain):
+    """The sweep's outcome against the default setting timed before (base) and after
+    (base2) it: the defaults, at the better of their two rates, unless the best setting
+    beats that rate by more than min_gain and by more than the spread of the two."""
+    noise = abs(base2 - base) / min(base, base2)
+    if best == DEFAULTS or best_rate < max(base, base2) * (1.0 + max(min_gain, noise)):
+        return dict(DEFAULTS), max(base, base2), noise
+    return best, best_rate, noise
+
+
 def tune(descs, step_tau=8.0, reps=3, device=0, cache=None, use_cache=True, knobs=None, min_gain=0.03):
     """Options for the workload `descs` (a list of Engine.add_ensemble keyword dicts).
     Returns dict(key, best, default_rate, best_rate, noise, trials); applies nothing.
@@ -146,9 +156,7 @@ def tune(descs, step_tau=8.0, reps=3, device=0, cache=None, use_cache=True, knob
             e.set_option(k, v)
         base2, t = _rate(e, ids, t, step_tau, reps)
         trials.append(dict(DEFAULTS, rate=base2, note="default, timed again after the sweep"))
-    noise = abs(base2 - base) / min(base, base2)
-    if best != DEFAULTS and best_rate < max(base, base2) * (1.0 + max(min_gain, noise)):
-        best, best_rate = dict(DEFAULTS), max(base, base2)  # within the noise: keep the defaults
+    best, best_rate, noise = _settle(best, best_rate, base, base2, min_gain)
     gain = 1.0 + max(min_gain, noise)
     if "window" in knobs:
         for v in CANDIDATES["window"]:

@@ -327,7 +327,7 @@ struct Ens {
 struct EngineOptions {
   int plain_jumps = 1;                   // DSM_OPT_PLAIN_JUMPS
   int small_per_cu = 12;                 // DSM_OPT_SMALL_PER_CU
-  int split_margin_div = 8;              // DSM_OPT_SPLIT_MARGIN
+  int split_margin_div = 16;             // DSM_OPT_SPLIT_MARGIN
   int lds_table_floats = kLdsTableFloats;  // DSM_OPT_LDS_TABLE_FLOATS
   int concurrent = -1;                   // DSM_OPT_CONCURRENT
   int blocks_per_cu = 0;                 // DSM_OPT_BLOCKS_PER_CU
@@ -731,7 +731,8 @@ int late_last(int bin_max) { return std::max(bin_max, kLateFirst); }
 // stream, the largest first (its few long chains take their slots before the bulk fills
 // the device), so the latency-bound runs of long chains overlap the bulk instead of
 // following it.  Chains that outgrow their geometry during this round go to late lists
-// that a second, sequential round runs through the larger geometries.  Every chain makes
+// that a second round runs through the larger geometries (the first of them behind the
+// row and pair forms, the others in order after the whole first round).  Every chain makes
 // the same jumps to the same targets as in the sequential order.
 int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   int* cnt = e->d_cnt;
@@ -759,13 +760,19 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // waves -- each needs a whole SIMD's registers -- are dispatched before the other
   // queues, released by the split's event, fill the device (launched on a side stream
   // they waited for a free SIMD: 140-180 ms for a 68 ms run).  G = 4 (also latency-bound)
-  // then the G = 2 bulk on the second stream, the merged row and pair forms on the
-  // third, the G = 1 bulk on the fourth.  The engine stream then runs a second launch of
-  // the G = 2 and the G = 1 bins: it shares their work queues, so whichever stream is
-  // free first takes the remaining chains of those bins (a launch that finds its queue
-  // empty exits at once).
+  // then on the second stream, the G = 1 bulk behind it, the merged row and pair forms on
+  // the third stream, the G = 2 bulk on the fourth.  The engine stream then runs second
+  // launches of the G = 2 and G = 1 bins, which share their work queues, so whichever
+  // stream is free first takes the remaining chains (a launch that finds its queue empty
+  // exits at once).  The G = 2 runs, longer than the G = 1 ones, start with the pass.
+  // Measured on the 262,144-chain melt, per 16 tau_K window: 111-113 ms; 118-123 ms with
+  // the G = 1 bulk at the start and G = 2 behind G = 8; 128-136 ms with both bins in one
+  // launch, G = 2 first.  The G = 4 waves (304 registers) find few SIMDs with room beside
+  // the bulk's persistent waves and run their chains a few after one another (110 ms for
+  // runs of 21 ms); holding the bulk streams back 100 us lets them all start at once and
+  // end at 21 ms, but the bulk then runs at a lower occupancy meanwhile: 114 ms.
   auto stream = [&](int k) { return k == 0 ? e->st : e->xs[k]; };  // xs[0] is not a stream
-  // bin 2 (128 <= Z < 224 at the split) runs two chains per wave in the two-group pair
+  // bin 2 (128 <= Z < 240 at the split) runs two chains per wave in the two-group pair
   // form when DSM_OPT_WIDE_PAIR is set (a third fewer VALU instructions per jump, the
   // same window time on the 262,144-chain melt: 129.1 against 129.8 ms); the late round
   // keeps one chain per wave
@@ -779,8 +786,8 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   HIPCHK(first(5, 0));
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->xs[k], e->ev_split, 0));
   HIPCHK(first(4, 1));
-  HIPCHK(first(3, 1));
-  HIPCHK(first(2, 3));
+  HIPCHK(first(2, 1));
+  HIPCHK(first(3, 3));
   bind(kSplitList + 1, late_of(1));
   const KmcArgs pair_args = a;
   bind(kSplitList, late_of(0));
@@ -789,19 +796,25 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   if (timed) HIPCHK(hipEventRecord(e->cev[1], e->xs[2]));
   HIPCHK(launch_kmc_rowpair(a, pair_args, std::max(nb, 1) * e->n_cu, e->xs[2]));
   if (timed) HIPCHK(hipEventRecord(e->cev[2], e->xs[2]));
-  ++rs->kmc_launches;
+  // the row and pair forms are the only feeders of the first late list, complete when
+  // their launch ends: its run follows on the same stream, beside the other geometries'
+  // bulk, instead of after the whole first round (118-123 ms per window against 123-126
+  // with the earlier stream layout)
+  bind(kLateList + kLateFirst, late_of(kLateFirst));
+  HIPCHK(launch_kmc(a, kLateFirst, false, grid(kLateFirst), e->xs[2]));
+  rs->kmc_launches += 2;
   HIPCHK(first(3, 0));
   HIPCHK(first(2, 0));
   for (int k = 1; k < 4; ++k) HIPCHK(hipEventRecord(e->ev_bin[k], e->xs[k]));
   for (int k = 1; k < 4; ++k) HIPCHK(hipStreamWaitEvent(e->st, e->ev_bin[k], 0));
   // late launches: chains that outgrew their geometry, through the larger ones in order
-  for (int b = kLateFirst; b <= late_last(ps.bin_max); ++b) {
+  for (int b = kLateFirst + 1; b <= late_last(ps.bin_max); ++b) {
     bind(kLateList + b, late_of(b));
     HIPCHK(launch_kmc(a, b, false, grid(b), e->st));
   }
   if (timed) HIPCHK(hipEventRecord(e->cev[3], e->st));
   ps.pending = timed ? 2 : 0;  // read by pass_times after the caller's next synchronisation
-  rs->kmc_launches += late_last(ps.bin_max) - kLateFirst + 1;
+  rs->kmc_launches += late_last(ps.bin_max) - kLateFirst;
   rs->first_launches++;
   return DSM_OK;
 }

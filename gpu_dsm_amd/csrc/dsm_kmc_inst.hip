@@ -6,7 +6,7 @@
 //   part of its state in scratch memory; it carries the rare chains with Z > 1024)
 //   6: the row and pair forms in one launch (batched, concurrent passes)
 //   7: pair form with two register groups: two chains per wave, Z <= 256 (the bulk of
-//      128 <= Z < 224 in concurrent passes)
+//      128 <= Z < 240 in concurrent passes)
 #include "dsm_kmc.h"
 #include "dsm_launch.h"
 

@@ -99,14 +99,14 @@ int dsm_mem_info(int32_t device_id, size_t* free_bytes, size_t* total_bytes);
 #define DSM_OPT_PLAIN_JUMPS 1       /* plain form: jumps per chain per launch (default 1, the reference's) */
 #define DSM_OPT_SMALL_PER_CU 2      /* passes of at most this many chains per CU run in the wave form (12) */
 #define DSM_OPT_SPLIT_MARGIN 3      /* concurrent pass: chains within capacity/value of a wave-form geometry
-                                       start in the next one (8; 0 = none) */
+                                       start in the next one (16; 0 = none) */
 #define DSM_OPT_LDS_TABLE_FLOATS 4  /* rate tables go to LDS when they fit this many floats per wave (2816) */
 #define DSM_OPT_CONCURRENT 5        /* geometries side by side: -1 by the pass's lengths, 0 never, 1 always */
 #define DSM_OPT_BLOCKS_PER_CU 6     /* cap on persistent waves per CU (0 = the occupancy limit) */
 #define DSM_OPT_WINDOW 7            /* samples per staging window of ensembles added later (256) */
 #define DSM_OPT_GRAPH 8             /* 1: batched windows run as captured HIP graphs, replayed with the
                                        window's parameters in a pinned block (0) */
-#define DSM_OPT_WIDE_PAIR 9         /* concurrent pass: chains with 128 <= Z < 224 run two per wave (1) or
+#define DSM_OPT_WIDE_PAIR 9         /* concurrent pass: chains with 128 <= Z < 240 run two per wave (1) or
                                        one per wave (0) (0) */
 int dsm_set_option(dsm_engine* e, int32_t key, int32_t value);
 int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value);

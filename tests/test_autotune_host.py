@@ -24,3 +24,20 @@ def test_cache_roundtrip(tmp_path):
     A._store(path, "k2", dict(best=dict(A.DEFAULTS, window=64), default_rate=1.0, best_rate=1.2))
     d = A._load(path)
     assert set(d) == {"k1", "k2"} and d["k2"]["best"]["window"] == 64
+
+
+def test_settle_keeps_defaults_at_their_better_rate():
+    from gpu_dsm_amd import autotune as A
+    other = dict(A.DEFAULTS, split_margin=0)
+    # the sweep found nothing better: the defaults, at the better of their two timings
+    best, rate, noise = A._settle(dict(A.DEFAULTS), 100.0, 100.0, 101.0, 0.03)
+    assert best == A.DEFAULTS and rate == 101.0 and abs(noise - 0.01) < 1e-12
+    # a setting within the gain threshold of the defaults does not replace them
+    best, rate, _ = A._settle(other, 102.0, 100.0, 101.0, 0.03)
+    assert best == A.DEFAULTS and rate == 101.0
+    # a clear gain, above both the threshold and the run-to-run spread, does
+    best, rate, _ = A._settle(other, 110.0, 100.0, 101.0, 0.03)
+    assert best == other and rate == 110.0
+    # ... unless the defaults' own two timings spread by more than that gain
+    best, rate, _ = A._settle(other, 110.0, 100.0, 108.0, 0.03)
+    assert best == A.DEFAULTS and rate == 108.0
