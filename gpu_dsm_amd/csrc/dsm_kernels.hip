@@ -650,9 +650,12 @@ __global__ void copy_probe_kernel(const float4* __restrict__ in, float4* __restr
 // 2/3/4 wave form G = 1/2/4, 7 pair form with two groups): in[4i..4i+3] = Qx Qy Qz N of
 // strand i, in[4n] = Nk; out = 3 floats.
 template <int CPW, int G>
-__device__ void com_probe(const float* in, float* out, int n) {
+__device__ void com_probe(const float* in, float* out, int n, lds_f* buf) {
+  static_assert(Geo<CPW, G>::STG <= 1024, "the strand lengths fit the probe's LDS buffer");
   const LaneC<CPW> L;
+  const Stage<CPW, G> S{buf, L.srow * G};
   Strands<G> C;
+  int nv[4 * G];
 #pragma unroll
   for (int j = 0; j < 4 * G; ++j) {
     const int p = L.pos(j);
@@ -660,20 +663,25 @@ __device__ void com_probe(const float* in, float* out, int n) {
     C.qx[j] = on ? in[4 * p] : 0.0f;
     C.qy[j] = on ? in[4 * p + 1] : 0.0f;
     C.qz[j] = on ? in[4 * p + 2] : 0.0f;
-    C.n[j] = on ? (int)in[4 * p + 3] : 0;
+    nv[j] = on ? (int)in[4 * p + 3] : 0;
     C.w[j] = 0.0f;
   }
+#pragma unroll
+  for (int g = 0; g < G; ++g) S.put4i(kStN, g, L.pos0, nv[4 * g], nv[4 * g + 1], nv[4 * g + 2], nv[4 * g + 3]);
+  wave_lds_sync();
   float sv[3];
-  com_sample<CPW, G>(C, n, in[4 * n], L, sv);
+  com_sample<CPW, G>(C, S, n, in[4 * n], L, sv);
   if (L.lane == 0) { out[0] = sv[0]; out[1] = sv[1]; out[2] = sv[2]; }
 }
 __global__ void __launch_bounds__(64) com_probe_kernel(const float* in, float* out, int n, int geom) {
-  if (geom == 0) com_probe<4, 1>(in, out, n);
-  else if (geom == 1) com_probe<2, 1>(in, out, n);
-  else if (geom == 2) com_probe<1, 1>(in, out, n);
-  else if (geom == 3) com_probe<1, 2>(in, out, n);
-  else if (geom == kGeomPair2) com_probe<2, 2>(in, out, n);
-  else com_probe<1, 4>(in, out, n);
+  __shared__ float buf[1024];  // the kStN staging array of one wave (strand lengths)
+  lds_f* b = (lds_f*)buf;
+  if (geom == 0) com_probe<4, 1>(in, out, n, b);
+  else if (geom == 1) com_probe<2, 1>(in, out, n, b);
+  else if (geom == 2) com_probe<1, 1>(in, out, n, b);
+  else if (geom == 3) com_probe<1, 2>(in, out, n, b);
+  else if (geom == kGeomPair2) com_probe<2, 2>(in, out, n, b);
+  else com_probe<1, 4>(in, out, n, b);
 }
 
 __global__ void __launch_bounds__(64) scan_probe_kernel(const float* in, float* out, int n, int form) {

@@ -491,6 +491,7 @@ struct Stage {
   __device__ __forceinline__ float getf(int a, int p) const { return arr(a)[p]; }
   __device__ __forceinline__ int geti(int a, int p) const { return ((lds_i*)arr(a))[p]; }
   __device__ __forceinline__ void putf(int a, int p, float v) const { arr(a)[p] = v; }
+  __device__ __forceinline__ void puti(int a, int p, int v) const { ((lds_i*)arr(a))[p] = v; }
   // this lane's four slots of register group g
   __device__ __forceinline__ void put4(int a, int g, int pos0, float v0, float v1, float v2, float v3) const {
     const vf4 v = {v0, v1, v2, v3};
@@ -503,7 +504,20 @@ struct Stage {
   __device__ __forceinline__ vf4 get4(int a, int g, int pos0) const {
     return *(lds_f4*)(arr(a) + Geo<CPW, G>::GS * g + pos0);
   }
+  __device__ __forceinline__ vi4 get4i(int a, int g, int pos0) const {
+    return *(lds_i4*)((lds_i*)arr(a) + Geo<CPW, G>::GS * g + pos0);
+  }
 };
+
+// this lane's strand lengths N (all register groups) from the staging array
+template <int CPW, int G>
+__device__ __forceinline__ void get_n(const Stage<CPW, G>& S, const LaneC<CPW>& L, int (&n)[4 * G]) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const vi4 v = S.get4i(kStN, g, L.pos0);
+    n[4 * g] = v.x; n[4 * g + 1] = v.y; n[4 * g + 2] = v.z; n[4 * g + 3] = v.w;
+  }
+}
 
 // orders this wave's LDS writes before its later reads (cross-lane exchange)
 __device__ __forceinline__ void wave_lds_sync() {
@@ -516,9 +530,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 template <int G>
 struct Strands {
   // (|Q|^2 of a strand is recomputed from qx, qy, qz where it is needed: the same FP32
-  // expression every time, so no register copy of it is kept)
+  // expression every time, so no register copy of it is kept.  The strand lengths N live
+  // in the LDS staging array kStN only: a shift changes two of them, which one lane
+  // writes there, and every reader -- end rates, sub-event, pair rates, samples, the
+  // write-back -- reads them from LDS.)
   float qx[4 * G], qy[4 * G], qz[4 * G], w[4 * G];
-  int n[4 * G];
 };
 
 // Chain-uniform values, replicated over the chain's lanes.
@@ -575,13 +591,6 @@ __device__ __forceinline__ void refill_rand(RandCache& R, uint64_t ev, int sl, u
 }
 
 // ------------------------------------------------------------------ building blocks
-// Stage the strand arrays used by the rate refresh and end rates.
-template <int CPW, int G>
-__device__ __forceinline__ void stage_n(const Stage<CPW, G>& S, const Strands<G>& C, const LaneC<CPW>& L) {
-#pragma unroll
-  for (int g = 0; g < G; ++g)
-    S.put4i(kStN, g, L.pos0, C.n[4 * g], C.n[4 * g + 1], C.n[4 * g + 2], C.n[4 * g + 3]);
-}
 // Stage |Q|^2 and the CD rates; q2 receives the |Q|^2 values for a refresh_all that
 // follows.
 template <int CPW, int G>
@@ -606,13 +615,15 @@ __device__ __forceinline__ void stage_qw(const Stage<CPW, G>& S, const Strands<G
 template <int CPW, int G, bool LT>
 __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strands<G>& C, const float (&q2)[4 * G],
                                             const Tabs<LT>& T, int cd, int Z, const LaneC<CPW>& L) {
+  int n[4 * G];
+  get_n<CPW, G>(S, L, n);
   int nn[G];
   float qn[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int gn = g + 1 < G ? g + 1 : g;
     const bool cross = g + 1 < G;
-    nn[g] = cross ? cross_next<CPW, int>(C.n[4 * g], C.n[4 * gn], L) : seg_next<CPW, int>(C.n[4 * g]);
+    nn[g] = cross ? cross_next<CPW, int>(n[4 * g], n[4 * gn], L) : seg_next<CPW, int>(n[4 * g]);
     qn[g] = cross ? cross_next<CPW, float>(q2[4 * g], q2[4 * gn], L) : seg_next<CPW, float>(q2[4 * g]);
   }
 #pragma unroll
@@ -622,10 +633,10 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
     for (int k = 0; k < 4; ++k) {
       const int j = 4 * g + k;
       const int p = L.pos(j);
-      const int Nj = (k < 3) ? C.n[j + (k < 3 ? 1 : 0)] : nn[g];
+      const int Nj = (k < 3) ? n[j + (k < 3 ? 1 : 0)] : nn[g];
       const float q2j = (k < 3) ? q2[j + (k < 3 ? 1 : 0)] : qn[g];
       wp[k] = 0.0f; wm[k] = 0.0f; rr[k] = 0.0f;
-      if (p <= Z - 2) pair_rate<LT>(T, cd, C.n[j], Nj, q2[j], q2j, C.w[j], p == 0, p + 1 == Z - 1, wp[k], wm[k], rr[k]);
+      if (p <= Z - 2) pair_rate<LT>(T, cd, n[j], Nj, q2[j], q2j, C.w[j], p == 0, p + 1 == Z - 1, wp[k], wm[k], rr[k]);
     }
     S.put4(kStWP, g, L.pos0, wp[0], wp[1], wp[2], wp[3]);
     S.put4(kStWM, g, L.pos0, wm[0], wm[1], wm[2], wm[3]);
@@ -695,6 +706,17 @@ __device__ __forceinline__ void gap_lds(const Stage<CPW, G>& S, int dir, int at,
   }
 }
 
+// Gap op on the LDS-resident strand lengths.
+template <int CPW, int G>
+__device__ __forceinline__ void gap_n(const Stage<CPW, G>& S, int dir, int at, int Z, const LaneC<CPW>& L) {
+  int v[4 * G];
+  get_n<CPW, G>(S, L, v);
+  if (dir > 0) gap_open<CPW, G>(v, at, Z, L);
+  else gap_close<CPW, G>(v, at, Z, L);
+#pragma unroll
+  for (int g = 0; g < G; ++g) S.put4i(kStN, g, L.pos0, v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3]);
+}
+
 // Flow: explicit Euler affine step Q <- Q + dt*kappa*Q (common/ensemble_kernel.cu:105-110)
 template <int CPW, int G>
 __device__ __forceinline__ void deform(Strands<G>& C, const EnsParams* P, float dt, int Z, const LaneC<CPW>& L) {
@@ -714,13 +736,15 @@ __device__ __forceinline__ void deform(Strands<G>& C, const EnsParams* P, float 
 
 // EQ sample (sxy, syz, sxz) = -sum 3 Qa Qb / N  (common/eq_ensemble_kernel.cu:281-296)
 template <int CPW, int G, bool LT>
-__device__ __forceinline__ void eq_sample(const Strands<G>& C, const Tabs<LT>& T, int Z, const LaneC<CPW>& L,
-                                          float out[3]) {
+__device__ __forceinline__ void eq_sample(const Strands<G>& C, const Stage<CPW, G>& S, const Tabs<LT>& T, int Z,
+                                          const LaneC<CPW>& L, float out[3]) {
+  int n[4 * G];
+  get_n<CPW, G>(S, L, n);
   float a[4 * G], b[4 * G], d[4 * G];
 #pragma unroll
   for (int j = 0; j < 4 * G; ++j) {
     const bool on = L.pos(j) < Z;
-    const float t3 = on ? T.T3(C.n[j]) : 0.0f;
+    const float t3 = on ? T.T3(n[j]) : 0.0f;
     a[j] = on ? (C.qx[j] * C.qy[j]) * t3 : 0.0f;
     b[j] = on ? (C.qy[j] * C.qz[j]) * t3 : 0.0f;
     d[j] = on ? (C.qx[j] * C.qz[j]) * t3 : 0.0f;
@@ -732,8 +756,10 @@ __device__ __forceinline__ void eq_sample(const Strands<G>& C, const Tabs<LT>& T
 
 // flow sample: xx yy zz xy yz zx Lpp Z (common/ensemble_kernel.cu:504-538)
 template <int CPW, int G, bool LT>
-__device__ __forceinline__ void flow_sample(const Strands<G>& C, const Tabs<LT>& T, int Z, const LaneC<CPW>& L,
-                                            float out[8]) {
+__device__ __forceinline__ void flow_sample(const Strands<G>& C, const Stage<CPW, G>& S, const Tabs<LT>& T, int Z,
+                                            const LaneC<CPW>& L, float out[8]) {
+  int n[4 * G];
+  get_n<CPW, G>(S, L, n);
 #pragma unroll
   for (int comp = 0; comp < 7; ++comp) {
     float v[4 * G];
@@ -741,7 +767,7 @@ __device__ __forceinline__ void flow_sample(const Strands<G>& C, const Tabs<LT>&
     for (int j = 0; j < 4 * G; ++j) {
       const bool on = L.pos(j) < Z;
       const float x = C.qx[j], y = C.qy[j], z = C.qz[j];
-      const float t3 = on ? T.T3(C.n[j]) : 0.0f;
+      const float t3 = on ? T.T3(n[j]) : 0.0f;
       float val;
       switch (comp) {
         case 0: val = (x * x) * t3; break;
@@ -764,8 +790,10 @@ __device__ __forceinline__ void flow_sample(const Strands<G>& C, const Tabs<LT>&
 // sum_i (P_i + Q_i/2) N_i / Nk with P_i the canonical inclusive scan of Q at i-1;
 // the caller adds R1.
 template <int CPW, int G>
-__device__ __forceinline__ void com_sample(const Strands<G>& C, int Z, float nkf, const LaneC<CPW>& L,
-                                           float out[3]) {
+__device__ __forceinline__ void com_sample(const Strands<G>& C, const Stage<CPW, G>& S, int Z, float nkf,
+                                           const LaneC<CPW>& L, float out[3]) {
+  int n[4 * G];
+  get_n<CPW, G>(S, L, n);
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
     float q[4 * G];
@@ -779,7 +807,7 @@ __device__ __forceinline__ void com_sample(const Strands<G>& C, int Z, float nkf
     shift_prev<CPW, G>(I, P, L);
 #pragma unroll
     for (int j = 0; j < 4 * G; ++j)
-      v[j] = L.pos(j) < Z ? ((P[j] + q[j] * 0.5f) * (float)C.n[j]) / nkf : 0.0f;
+      v[j] = L.pos(j) < Z ? ((P[j] + q[j] * 0.5f) * (float)n[j]) / nkf : 0.0f;
     out[comp] = chain_total<CPW, G>(v, Z, L);
   }
 }
@@ -788,8 +816,9 @@ __device__ __forceinline__ void com_sample(const Strands<G>& C, int Z, float nkf
 // t > k*delta (the state after the first jump past k*delta); flow output k while
 // t >= k*delta.
 template <int CPW, int G, bool LT, bool EXT>
-__device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<LT>& T, const EnsLocal& E, Hdr& H,
-                                               float* stage, const float* r1, const LaneC<CPW>& L) {
+__device__ __forceinline__ void record_samples(const Strands<G>& C, const Stage<CPW, G>& S, const Tabs<LT>& T,
+                                               const EnsLocal& E, Hdr& H, float* stage, const float* r1,
+                                               const LaneC<CPW>& L) {
   if (E.sample_mode == kSampleEq || (EXT && E.sample_mode == kSampleMsd)) {
     const bool msd = EXT && E.sample_mode == kSampleMsd;
     for (;;) {
@@ -797,10 +826,10 @@ __device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<L
       if (!__ballot(pend)) break;
       float sv[3];
       if constexpr (EXT) {
-        if (msd) com_sample<CPW, G>(C, H.Z, (float)H.nk, L, sv);
-        else eq_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+        if (msd) com_sample<CPW, G>(C, S, H.Z, (float)H.nk, L, sv);
+        else eq_sample<CPW, G, LT>(C, S, T, H.Z, L, sv);
       } else {
-        eq_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+        eq_sample<CPW, G, LT>(C, S, T, H.Z, L, sv);
       }
       if (pend) {
         if (L.sl == 0 && H.next_sample >= E.k0) {
@@ -821,7 +850,7 @@ __device__ __forceinline__ void record_samples(const Strands<G>& C, const Tabs<L
       const bool pend = !H.inert && H.t >= H.t_next;
       if (!__ballot(pend)) break;
       float sv[8];
-      flow_sample<CPW, G, LT>(C, T, H.Z, L, sv);
+      flow_sample<CPW, G, LT>(C, S, T, H.Z, L, sv);
       if (pend) {
         if (L.sl == 0 && H.next_sample >= E.k0) {
           float* o = stage + E.stage_off + ((H.next_sample - E.k0) * E.nchains + H.local) * 8;
@@ -955,14 +984,14 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
       }
     }
   }
-  // 4. effect.  Common case: an N shift across pair p.
+  // 4. effect.  Common case: an N shift across pair p (p <= Z - 2 < CAP - 2, so Np and
+  // Np1 above are N at p and p + 1): the chain's first lane writes the two new lengths.
   int lo = 1, hi = 0, zero_at = -1;
   if (kind == kEvShiftP || kind == kEvShiftM) {
     const int dn = (kind == kEvShiftP) ? 1 : -1;
-#pragma unroll
-    for (int j = 0; j < 4 * G; ++j) {
-      const int d = L.pos(j) - p;
-      C.n[j] += (d == 0) ? dn : ((d == 1) ? -dn : 0);
+    if (L.sl == 0) {
+      S.puti(kStN, p, Np + dn);
+      S.puti(kStN, p + 1, Np1 - dn);
     }
     lo = p - 1; hi = p + 1;
     H.dlo = min(H.dlo, p); H.dhi = max(H.dhi, p + 1);
@@ -1085,28 +1114,36 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
         H.dlo = min(H.dlo, Z - 2); H.dhi = max(H.dhi, Z - 2);
       }
     }
-    // gaps (whole wave; identity on chains without one)
+    // gaps (whole wave; identity on chains without one).  The lengths move in LDS, where
+    // the other chains of the wave may have just written a shift: order those writes first.
+    wave_lds_sync();
     const bool any_open = __ballot(gap > 0) != 0, any_close = __ballot(gap < 0) != 0;
     if (any_open) {
       const int ao = gap > 0 ? at : CAP;
       gap_open<CPW, G>(C.qx, ao, Z, L); gap_open<CPW, G>(C.qy, ao, Z, L); gap_open<CPW, G>(C.qz, ao, Z, L);
-      gap_open<CPW, G>(C.w, ao, Z, L); gap_open<CPW, G>(C.n, ao, Z, L);
+      gap_open<CPW, G>(C.w, ao, Z, L);
+      gap_n<CPW, G>(S, 1, ao, Z, L);
       if (!EXT || !E.flow) gap_lds<CPW, G>(S, 1, ao, Z, L);
     }
     if (any_close) {
       const int ac = gap < 0 ? at : CAP;
       gap_close<CPW, G>(C.qx, ac, Z, L); gap_close<CPW, G>(C.qy, ac, Z, L); gap_close<CPW, G>(C.qz, ac, Z, L);
-      gap_close<CPW, G>(C.w, ac, Z, L); gap_close<CPW, G>(C.n, ac, Z, L);
+      gap_close<CPW, G>(C.w, ac, Z, L);
+      gap_n<CPW, G>(S, -1, ac, Z, L);
       if (!EXT || !E.flow) gap_lds<CPW, G>(S, -1, ac, Z, L);
     }
-    // patches
+    // patches (the lengths by the chain's first lane, after the gap ops' LDS writes)
 #pragma unroll
     for (int j = 0; j < 4 * G; ++j) {
       const int ps = L.pos(j);
-      if (ps == p0) { C.n[j] = p0N; C.qx[j] = p0x; C.qy[j] = p0y; C.qz[j] = p0z; }
-      if (ps == p1) { C.n[j] = p1N; C.qx[j] = p1x; C.qy[j] = p1y; C.qz[j] = p1z; }
+      if (ps == p0) { C.qx[j] = p0x; C.qy[j] = p0y; C.qz[j] = p0z; }
+      if (ps == p1) { C.qx[j] = p1x; C.qy[j] = p1y; C.qz[j] = p1z; }
       if (ps == w0i) C.w[j] = w0;
       if (ps == w1i) C.w[j] = w1;
+    }
+    if (L.sl == 0) {
+      if (p0 >= 0) S.puti(kStN, p0, p0N);
+      if (p1 >= 0) S.puti(kStN, p1, p1N);
     }
     // MSD: a first entanglement created before the old one moves R1 back by the new Q_1
     // (common/eq_ensemble_kernel.cu:531-539,606-614)
@@ -1122,7 +1159,6 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   // so the EXT variant, which the passes with a flow ensemble launch, neither shifts the
   // cached rates (gap_lds above) nor patches them here; the lean variant keeps the
   // unconditional code, measured 1.3% faster for equilibrium passes.
-  stage_n<CPW, G>(S, C, L);
   wave_lds_sync();
   if (!EXT || !E.flow) {
     refresh_pairs<CPW, G, LT>(S, T, E.cd, H.Z, lo, hi, zero_at, L);
@@ -1132,8 +1168,8 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
 
 // ------------------------------------------------------------------ load / store
 template <int CPW, int G>
-__device__ __forceinline__ void load_strands(Strands<G>& C, const ChainArrays& ch, const Hdr& H, int64_t base,
-                                             const LaneC<CPW>& L) {
+__device__ __forceinline__ void load_strands(Strands<G>& C, const Stage<CPW, G>& S, const ChainArrays& ch,
+                                             const Hdr& H, int64_t base, const LaneC<CPW>& L) {
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int p0 = Geo<CPW, G>::GS * g + L.pos0;
@@ -1147,21 +1183,25 @@ __device__ __forceinline__ void load_strands(Strands<G>& C, const ChainArrays& c
     const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w}, zs[4] = {z.x, z.y, z.z, z.w};
     const float ws[4] = {w.x, w.y, w.z, w.w};
     const int ns[4] = {n.x, n.y, n.z, n.w};
+    int nv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int j = 4 * g + k;
       const bool on = p0 + k < H.Z;
       C.qx[j] = on ? xs[k] : 0.0f; C.qy[j] = on ? ys[k] : 0.0f; C.qz[j] = on ? zs[k] : 0.0f;
-      C.w[j] = on ? ws[k] : 0.0f; C.n[j] = on ? ns[k] : 0;
+      C.w[j] = on ? ws[k] : 0.0f; nv[k] = on ? ns[k] : 0;
     }
+    S.put4i(kStN, g, L.pos0, nv[0], nv[1], nv[2], nv[3]);
   }
 }
 
 // positions lo..hi (and < Z) back to HBM; whole 4-slot groups when they lie inside
 template <int CPW, int G>
-__device__ __forceinline__ void store_strands(const Strands<G>& C, const ChainArrays& ch, const Hdr& H, int64_t base,
-                                              int lo, int hi, const LaneC<CPW>& L) {
+__device__ __forceinline__ void store_strands(const Strands<G>& C, const Stage<CPW, G>& S, const ChainArrays& ch,
+                                              const Hdr& H, int64_t base, int lo, int hi, const LaneC<CPW>& L) {
   hi = min(hi, H.Z - 1);
+  int n[4 * G];
+  get_n<CPW, G>(S, L, n);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int p0 = Geo<CPW, G>::GS * g + L.pos0;
@@ -1171,14 +1211,14 @@ __device__ __forceinline__ void store_strands(const Strands<G>& C, const ChainAr
       *(float4*)(ch.Qy + o) = make_float4(C.qy[4 * g], C.qy[4 * g + 1], C.qy[4 * g + 2], C.qy[4 * g + 3]);
       *(float4*)(ch.Qz + o) = make_float4(C.qz[4 * g], C.qz[4 * g + 1], C.qz[4 * g + 2], C.qz[4 * g + 3]);
       *(float4*)(ch.w + o) = make_float4(C.w[4 * g], C.w[4 * g + 1], C.w[4 * g + 2], C.w[4 * g + 3]);
-      *(int4*)(ch.N + o) = make_int4(C.n[4 * g], C.n[4 * g + 1], C.n[4 * g + 2], C.n[4 * g + 3]);
+      *(int4*)(ch.N + o) = make_int4(n[4 * g], n[4 * g + 1], n[4 * g + 2], n[4 * g + 3]);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int j = 4 * g + k;
         if (p0 + k >= lo && p0 + k <= hi) {
           ch.Qx[o + k] = C.qx[j]; ch.Qy[o + k] = C.qy[j]; ch.Qz[o + k] = C.qz[j];
-          ch.w[o + k] = C.w[j]; ch.N[o + k] = C.n[j];
+          ch.w[o + k] = C.w[j]; ch.N[o + k] = n[j];
         }
       }
     }
@@ -1239,8 +1279,7 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
   H.e1 = 1.0f / (E.beta * (float)H.nk);
   Stage<CPW, G> S{stg, L.srow * G};
   Strands<G> C;
-  load_strands<CPW, G>(C, a.ch, H, H.c >= 0 ? a.ch.base[H.c] : 0, L);
-  stage_n<CPW, G>(S, C, L);
+  load_strands<CPW, G>(C, S, a.ch, H, H.c >= 0 ? a.ch.base[H.c] : 0, L);
   {
     float q2[4 * G];
     stage_qw<CPW, G>(S, C, L, q2);
@@ -1262,7 +1301,7 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
       }
       H.dt_prev = 0.0f;
     }
-    record_samples<CPW, G, LT, EXT>(C, T, E, H, a.stage, a.ch.R1, L);
+    record_samples<CPW, G, LT, EXT>(C, S, T, E, H, a.stage, a.ch.R1, L);
     bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0 && !(PLAIN && H.done >= a.plain_jumps);
     if (act && H.Z >= CAP) { H.ovf = 1; act = false; }
     H.inert = H.inert || (H.ovf != 0);
@@ -1272,8 +1311,8 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
   // write back every chain that was loaded live
   if (H.c >= 0 && !inert0) {
     const int64_t base = a.ch.base[H.c];
-    if (PLAIN) store_strands<CPW, G>(C, a.ch, H, base, H.dlo, H.dhi, L);
-    else store_strands<CPW, G>(C, a.ch, H, base, 0, H.Z - 1, L);
+    if (PLAIN) store_strands<CPW, G>(C, S, a.ch, H, base, H.dlo, H.dhi, L);
+    else store_strands<CPW, G>(C, S, a.ch, H, base, 0, H.Z - 1, L);
     if (L.sl == 0) {
       const int c = H.c;
       a.ch.Z[c] = H.Z; a.ch.stall[c] = H.stall; a.ch.next_sample[c] = H.next_sample; a.ch.t[c] = H.t;
@@ -1317,7 +1356,9 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
   Stage<CPW, G> S{stg, L.srow * G};
   Strands<G> C;
 #pragma unroll
-  for (int j = 0; j < 4 * G; ++j) { C.qx[j] = 0.f; C.qy[j] = 0.f; C.qz[j] = 0.f; C.w[j] = 0.f; C.n[j] = 0; }
+  for (int j = 0; j < 4 * G; ++j) { C.qx[j] = 0.f; C.qy[j] = 0.f; C.qz[j] = 0.f; C.w[j] = 0.f; }
+#pragma unroll
+  for (int g = 0; g < G; ++g) S.put4i(kStN, g, L.pos0, 0, 0, 0, 0);
   RandCache RC{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int k_rng = SEG;
   for (;;) {
@@ -1372,9 +1413,9 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
         if (dead) H.Z = 1;
         H.t_next = next_time(E, H.next_sample);
         H.e1 = 1.0f / (E.beta * (float)H.nk);
-        load_strands<CPW, G>(C, a.ch, H, a.ch.base[c], L);
+        load_strands<CPW, G>(C, S, a.ch, H, a.ch.base[c], L);
       }
-      stage_n<CPW, G>(S, C, L);
+      wave_lds_sync();
       {
         float q2[4 * G];
         stage_qw<CPW, G>(S, C, L, q2);
@@ -1400,7 +1441,7 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
         }
         H.dt_prev = 0.0f;
       }
-      record_samples<CPW, G, LT, EXT>(C, T, E, H, a.stage, a.ch.R1, L);
+      record_samples<CPW, G, LT, EXT>(C, S, T, E, H, a.stage, a.ch.R1, L);
       bool act = !H.inert && H.stall == 0 && H.t < E.t_end && H.left > 0;
       if (act && H.Z >= CAP) { H.ovf = 1; act = false; }
       if (H.ovf && !H.inert) H.inert = 1;
@@ -1411,7 +1452,7 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
     // 5. write back the finished chains and hand on the overflowed ones
     if (fin) {
       if (H.inert != kDeadAtLoad) {
-        store_strands<CPW, G>(C, a.ch, H, a.ch.base[H.c], 0, H.Z - 1, L);
+        store_strands<CPW, G>(C, S, a.ch, H, a.ch.base[H.c], 0, H.Z - 1, L);
         if (L.sl == 0) {
           const int c = H.c;
           a.ch.Z[c] = H.Z; a.ch.stall[c] = H.stall; a.ch.next_sample[c] = H.next_sample; a.ch.t[c] = H.t;
