@@ -757,7 +757,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // first launches.  A process has four hardware queues on this device, so four streams.
   // The longest chains (G = 8: few chains, a latency-bound run that sets the window's
   // critical path) go first, on the engine stream right behind the split, so their
-  // waves -- each needs a whole SIMD's registers -- are dispatched before the other
+  // waves -- each leaves no room for another wave on its SIMD -- are dispatched before the other
   // queues, released by the split's event, fill the device (launched on a side stream
   // they waited for a free SIMD: 140-180 ms for a 68 ms run).  G = 4 (also latency-bound)
   // then on the second stream, the G = 1 bulk behind it, the merged row and pair forms on
@@ -767,7 +767,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // exits at once).  The G = 2 runs, longer than the G = 1 ones, start with the pass.
   // Measured on the 262,144-chain melt, per 16 tau_K window: 111-113 ms; 118-123 ms with
   // the G = 1 bulk at the start and G = 2 behind G = 8; 128-136 ms with both bins in one
-  // launch, G = 2 first.  The G = 4 waves (304 registers) find few SIMDs with room beside
+  // launch, G = 2 first.  The G = 4 waves (260 registers) find few SIMDs with room beside
   // the bulk's persistent waves and run their chains a few after one another (110 ms for
   // runs of 21 ms); holding the bulk streams back 100 us lets them all start at once and
   // end at 21 ms, but the bulk then runs at a lower occupancy meanwhile: 114 ms.
