@@ -610,11 +610,52 @@ __device__ __forceinline__ void stage_qw(const Stage<CPW, G>& S, const Strands<G
   stage_qw<CPW, G>(S, C, L, q2);
 }
 
+// The five chain-end rates (common/eq_ensemble_kernel.cu:340-392) from the staged
+// strand lengths at both ends: CD creation in the tail strand, SD creation and
+// destruction at the head and the tail.  They change only when the lengths at a chain
+// end (N0, N1, N[Z-2], N[Z-1]) or Z do.  The pair and wave forms keep their sum in the
+// rate cache at position Z-1 and recompute it after a jump that changed one of those
+// (one to two chains per wave: a few percent of the jumps of a long chain).  The row
+// form, whose four short chains touch an end in most steps, computes it at every jump
+// and keeps the cache at Z-1 zero (measured on the 131,072-chain Nk = 100 melt: the row
+// kernel 8.9 ms that way against 9.1 ms cached; the 262,144-chain polydisperse melt's
+// window 98.5-99 ms against 102-104 ms with every form computing them per jump).
+template <int CPW>
+constexpr bool kEndCache = CPW <= 2;
+struct EndRates {
+  float cdcz, sdc1, sdcz, sdd1, sddz;
+  __device__ __forceinline__ float total() const { return (((cdcz + sdc1) + sdcz) + sdd1) + sddz; }
+};
+template <int CPW, int G, bool LT>
+__device__ __forceinline__ EndRates end_rates(const Stage<CPW, G>& S, const Tabs<LT>& T, int Z, float e1) {
+  const int N0 = S.geti(kStN, 0), N1 = S.geti(kStN, 1);
+  const int Nz = S.geti(kStN, Z - 1), Nz2 = S.geti(kStN, Z >= 2 ? Z - 2 : 0);
+  const bool z2 = (Z == 2);
+  const int ho = (N0 == 1) ? (z2 ? 6 : 5) * T.n + N1 : 4 * T.n + N0;
+  const int to = (Nz == 1) ? (z2 ? 6 : 5) * T.n + Nz2 : 4 * T.n + Nz;
+  const float h = T.at(ho), tt = T.at(to);
+  EndRates r;
+  r.cdcz = T.CC(Nz);
+  if (Z == 1) {
+    r.sdc1 = e1; r.sdcz = e1; r.sdd1 = 0.0f; r.sddz = 0.0f;
+  } else {
+    r.sdd1 = (N0 == 1) ? h : 0.0f;
+    r.sdc1 = (N0 == 1) ? 0.0f : h;
+    r.sddz = (Nz == 1) ? tt : 0.0f;
+    r.sdcz = (Nz == 1) ? 0.0f : tt;
+  }
+  return r;
+}
+
 // Recompute every cached pair rate of the chain (after a load or a deformation):
-// each lane evaluates the pairs of its own slots.
+// each lane evaluates the pairs of its own slots; the chain's first lane then puts the
+// chain-end rates at Z-1 (after the other lanes' lengths are visible).  Segments without
+// a live chain (empty, finished, stalled, overflowed) read nothing and cache zeros: their
+// lengths may be a finished chain's, of another ensemble than the tables.
 template <int CPW, int G, bool LT>
 __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strands<G>& C, const float (&q2)[4 * G],
-                                            const Tabs<LT>& T, int cd, int Z, const LaneC<CPW>& L) {
+                                            const Tabs<LT>& T, int cd, int Z, float e1, bool live,
+                                            const LaneC<CPW>& L) {
   int n[4 * G];
   get_n<CPW, G>(S, L, n);
   int nn[G];
@@ -636,11 +677,19 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
       const int Nj = (k < 3) ? n[j + (k < 3 ? 1 : 0)] : nn[g];
       const float q2j = (k < 3) ? q2[j + (k < 3 ? 1 : 0)] : qn[g];
       wp[k] = 0.0f; wm[k] = 0.0f; rr[k] = 0.0f;
-      if (p <= Z - 2) pair_rate<LT>(T, cd, n[j], Nj, q2[j], q2j, C.w[j], p == 0, p + 1 == Z - 1, wp[k], wm[k], rr[k]);
+      if (live && p <= Z - 2)
+        pair_rate<LT>(T, cd, n[j], Nj, q2[j], q2j, C.w[j], p == 0, p + 1 == Z - 1, wp[k], wm[k], rr[k]);
     }
     S.put4(kStWP, g, L.pos0, wp[0], wp[1], wp[2], wp[3]);
     S.put4(kStWM, g, L.pos0, wm[0], wm[1], wm[2], wm[3]);
     S.put4(kStRR, g, L.pos0, rr[0], rr[1], rr[2], rr[3]);
+  }
+  if constexpr (kEndCache<CPW>) {
+    wave_lds_sync();
+    if (live) {
+      const float eend = end_rates<CPW, G, LT>(S, T, Z, e1).total();
+      if (L.sl == 0) S.putf(kStRR, Z - 1, eend);
+    }
   }
 }
 
@@ -649,7 +698,8 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
 // instruction stream: lane j (0..2) evaluates W+ of pair lo+j and lane 8+j its W-
 // (the same FP operations as pair_rate, with the tables and signs chosen per lane);
 // lane j then adds the halves (W- arrives by DPP row_ror:8) and writes W+, W-, R.
-// Lane 3 zeroes the rate cache at zero_at (>= 0).
+// Lane 3 zeroes the rate cache at zero_at (>= 0): the old end position, past the end
+// after a destruction.
 template <int CPW, int G, bool LT>
 __device__ __forceinline__ void refresh_pairs(const Stage<CPW, G>& S, const Tabs<LT>& T, int cd, int Z, int lo, int hi,
                                               int zero_at, const LaneC<CPW>& L) {
@@ -876,35 +926,21 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
     refill_rand(RC, H.ev, L.sl, H.cid, E);
     k_rng = 0;
   }
-  // 1. rates at positions: cached pair rates, the chain-end rates at Z-1
+  // 1. rates at positions: the rate cache holds the pair rates at 0..Z-2 and (pair and
+  // wave forms) the sum of the chain-end rates at Z-1; the row form puts that sum there
   float v[4 * G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const vf4 x = S.get4(kStRR, g, L.pos0);
     v[4 * g] = x.x; v[4 * g + 1] = x.y; v[4 * g + 2] = x.z; v[4 * g + 3] = x.w;
   }
-  // chain-end rates (common/eq_ensemble_kernel.cu:340-392): cdcz sdc1 sdcz sdd1 sddz
-  const int N0 = S.geti(kStN, 0), N1 = S.geti(kStN, 1);
-  const int Nz = S.geti(kStN, Z - 1), Nz2 = S.geti(kStN, Z >= 2 ? Z - 2 : 0);
-  float cdcz, sdc1, sdcz, sdd1, sddz;
-  {
-    const bool z2 = (Z == 2);
-    const int ho = (N0 == 1) ? (z2 ? 6 : 5) * T.n + N1 : 4 * T.n + N0;
-    const int to = (Nz == 1) ? (z2 ? 6 : 5) * T.n + Nz2 : 4 * T.n + Nz;
-    const float h = T.at(ho), tt = T.at(to);
-    cdcz = T.CC(Nz);
-    if (Z == 1) {
-      sdc1 = H.e1; sdcz = H.e1; sdd1 = 0.0f; sddz = 0.0f;
-    } else {
-      sdd1 = (N0 == 1) ? h : 0.0f;
-      sdc1 = (N0 == 1) ? 0.0f : h;
-      sddz = (Nz == 1) ? tt : 0.0f;
-      sdcz = (Nz == 1) ? 0.0f : tt;
-    }
-  }
-  const float Eend = (((cdcz + sdc1) + sdcz) + sdd1) + sddz;
+  EndRates er_row{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if constexpr (!kEndCache<CPW>) {
+    er_row = end_rates<CPW, G, LT>(S, T, Z, H.e1);
+    const float Eend = er_row.total();
 #pragma unroll
-  for (int j = 0; j < 4 * G; ++j) v[j] = (L.pos(j) == Z - 1) ? Eend : v[j];
+    for (int j = 0; j < 4 * G; ++j) v[j] = (L.pos(j) == Z - 1) ? Eend : v[j];
+  }
   // 2. scan, total, clock
   float Gs[4 * G];
   const float W = chain_scan<CPW, G>(v, Gs, Z, L);
@@ -921,7 +957,13 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   const float pr = W * u_sel;
   const int p = pick_position<CPW, G>(Gs, pr, Z, L);
   wave_lds_sync();
-  // 3. sub-event (per lane, replicated per chain)
+  // 3. sub-event (per lane, replicated per chain); the pair and wave forms compute the
+  // chain-end rates one by one only where a chain picked its end position
+  EndRates er = er_row;
+  if constexpr (kEndCache<CPW>) {
+    if (__ballot(act && p == Z - 1)) er = end_rates<CPW, G, LT>(S, T, Z, H.e1);
+  }
+  const float cdcz = er.cdcz, sdc1 = er.sdc1, sdcz = er.sdcz, sdd1 = er.sdd1, sddz = er.sddz;
   const int pa = p < CAP - 2 ? p : CAP - 2;  // in-range position for the reads below
   const float Gprev = S.getf(kStG, pa > 0 ? pa - 1 : 0);
   const int Np = S.geti(kStN, pa), Np1 = S.geti(kStN, pa + 1);
@@ -987,7 +1029,10 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   // 4. effect.  Common case: an N shift across pair p (p <= Z - 2 < CAP - 2, so Np and
   // Np1 above are N at p and p + 1): the chain's first lane writes the two new lengths.
   int lo = 1, hi = 0, zero_at = -1;
+  // the chain-end rates change with N0, N1, N[Z-2], N[Z-1] and Z
+  bool ends = kind >= kEvCDd && kind <= kEvSDdTail;
   if (kind == kEvShiftP || kind == kEvShiftM) {
+    ends = p <= 1 || p >= Z - 3;
     const int dn = (kind == kEvShiftP) ? 1 : -1;
     if (L.sl == 0) {
       S.puti(kStN, p, Np + dn);
@@ -1018,6 +1063,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
       g2 = seg_get<CPW>(RC.g2, k_rng - 1, L);
     }
     const int N0n = S.geti(kStN, 0), N1n = S.geti(kStN, 1);
+    const int Nz = S.geti(kStN, Z - 1), Nz2 = S.geti(kStN, Z >= 2 ? Z - 2 : 0);
     // polydisperse background: the new link's partner quantile is the top bits of word 3
     // of this jump's Philox draw (H.ev already counts the jump)
     int pidx = 0;
@@ -1042,7 +1088,10 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
         p0 = e; p0N = Ne + Ne1; p0x = mx; p0y = my; p0z = mz;
         w0i = e; w0 = wcp1;
         lo = e - 1; hi = e;
-        zero_at = Z - 2;  // new last position (the gap close shifts nothing when e = Z-2)
+        // the old end position, past the end now (with the cached chain-end rates: the gap
+        // close leaves them there), or the new one (the gap close shifts nothing when
+        // e = Z-2)
+        zero_at = kEndCache<CPW> ? Z - 1 : Z - 2;
         H.dlo = min(H.dlo, e); H.dhi = max(H.dhi, Z - 1);
       } else if (kind == kEvCDc) {     // CD creation inside strand e
         const int Ne = Np;
@@ -1104,13 +1153,14 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
         gap = -1; at = 0; znew = Z - 1;
         p0 = 0; p0N = N1n + 1;
         lo = 0; hi = 0;
+        if (kEndCache<CPW>) zero_at = Z - 1;  // the gap close leaves the cached end rates there
         H.dlo = min(H.dlo, 0); H.dhi = max(H.dhi, Z - 1);
       } else {                          // SD destruction at the tail
         znew = Z - 1;
         p0 = Z - 2; p0N = Nz2 + 1;
         w0i = Z - 2; w0 = 0.0f;
         lo = Z - 3; hi = Z - 3;
-        zero_at = Z - 2;
+        zero_at = kEndCache<CPW> ? Z - 1 : Z - 2;  // the old (cached end rates) or new end position
         H.dlo = min(H.dlo, Z - 2); H.dhi = max(H.dhi, Z - 2);
       }
     }
@@ -1162,6 +1212,13 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   wave_lds_sync();
   if (!EXT || !E.flow) {
     refresh_pairs<CPW, G, LT>(S, T, E.cd, H.Z, lo, hi, zero_at, L);
+    // the chain-end rates at the (new) end position, after refresh_pairs' writes
+    if constexpr (kEndCache<CPW>) {
+      if (__ballot(ends)) {
+        const float eend = end_rates<CPW, G, LT>(S, T, H.Z, H.e1).total();
+        if (ends && L.sl == 0) S.putf(kStRR, H.Z - 1, eend);
+      }
+    }
     wave_lds_sync();
   }
 }
@@ -1283,7 +1340,7 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
   {
     float q2[4 * G];
     stage_qw<CPW, G>(S, C, L, q2);
-    refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
+    refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, H.e1, H.inert == 0, L);
   }
   wave_lds_sync();
   RandCache RC{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -1295,7 +1352,7 @@ __device__ __forceinline__ void run_unit(const KmcArgs& a, int unit, int n_entri
         if (dfm) deform<CPW, G>(C, E.P, H.dt_prev, H.Z, L);
         float q2[4 * G];
         stage_qw<CPW, G>(S, C, L, q2);
-        refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
+        refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, H.e1, H.inert == 0, L);
         wave_lds_sync();
         if (dfm) { H.dlo = 0; H.dhi = max(H.dhi, H.Z - 1); }
       }
@@ -1419,7 +1476,7 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
       {
         float q2[4 * G];
         stage_qw<CPW, G>(S, C, L, q2);
-        refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
+        refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, H.e1, H.inert == 0, L);
       }
       wave_lds_sync();
       // the cache is refilled at every chain's current jump count before the next jump
@@ -1436,7 +1493,7 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
           if (dfm) deform<CPW, G>(C, E.P, H.dt_prev, H.Z, L);
           float q2[4 * G];
           stage_qw<CPW, G>(S, C, L, q2);
-          refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, L);
+          refresh_all<CPW, G, LT>(S, C, q2, T, E.cd, H.Z, H.e1, H.inert == 0, L);
           wave_lds_sync();
         }
         H.dt_prev = 0.0f;
