@@ -49,18 +49,27 @@ typedef __attribute__((address_space(3))) vi4 lds_i4;
 template <bool LDS> struct TabPtr { typedef const __attribute__((address_space(1))) float* type; };
 template <> struct TabPtr<true> { typedef const lds_f* type; };
 
+// Table reads.  Global memory: a 32-bit byte offset from the wave-uniform base, so the
+// read takes the scalar-base form (one shift per index instead of a sign extension and
+// a 64-bit add); every index is a length or length sum, never negative.
+__device__ __forceinline__ float tab_ld(const lds_f* b, int i) { return b[i]; }
+__device__ __forceinline__ float tab_ld(const __attribute__((address_space(1))) float* b, int i) {
+  typedef const __attribute__((address_space(1))) char* gbytes;
+  return *(const __attribute__((address_space(1))) float*)((gbytes)b + (unsigned)i * 4u);
+}
+
 template <bool LDS>
 struct Tabs {
   typename TabPtr<LDS>::type b;
   int n;
-  __device__ __forceinline__ float at(int i) const { return b[i]; }
-  __device__ __forceinline__ float SP(int N) const { return b[N]; }
-  __device__ __forceinline__ float SM(int N) const { return b[n + N]; }
-  __device__ __forceinline__ float PP(int N) const { return b[2 * n + N]; }
-  __device__ __forceinline__ float PM(int N) const { return b[3 * n + N]; }
-  __device__ __forceinline__ float T3(int N) const { return b[7 * n + N]; }
-  __device__ __forceinline__ float CC(int N) const { return b[8 * n + N]; }
-  __device__ __forceinline__ float FR(int x) const { return b[9 * n + x]; }
+  __device__ __forceinline__ float at(int i) const { return tab_ld(b, i); }
+  __device__ __forceinline__ float SP(int N) const { return tab_ld(b, N); }
+  __device__ __forceinline__ float SM(int N) const { return tab_ld(b, n + N); }
+  __device__ __forceinline__ float PP(int N) const { return tab_ld(b, 2 * n + N); }
+  __device__ __forceinline__ float PM(int N) const { return tab_ld(b, 3 * n + N); }
+  __device__ __forceinline__ float T3(int N) const { return tab_ld(b, 7 * n + N); }
+  __device__ __forceinline__ float CC(int N) const { return tab_ld(b, 8 * n + N); }
+  __device__ __forceinline__ float FR(int x) const { return tab_ld(b, 9 * n + x); }
 };
 
 // CD lifetime samplers returning the rate 1/tau (common/gpu_random.cu:143-148)
