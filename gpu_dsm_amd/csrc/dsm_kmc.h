@@ -1539,10 +1539,10 @@ __host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G
 // PLAIN: a.plain_jumps jumps per chain, the state in HBM between launches (the
 // per-launch form; one jump per launch is the reference's structure); otherwise run to
 // the targets.  Both forms are held to the same occupancy targets: three waves per SIMD
-// (168 VGPRs) up to G = 2 (the G = 2 wave form spills 36 bytes per lane there, and still
-// finished the 262,144-chain melt's window in 117-126 ms against 130-131 ms at two waves:
-// its chains, each a serial run of about 2,000 jumps per window, need waves in flight),
-// one for G >= 4 (AGPRs as well).
+// (168 VGPRs) up to G = 2 (the G = 2 wave form spills 20 bytes per lane there; with 36
+// spilled it finished the 262,144-chain melt's window in 117-126 ms against 130-131 ms at
+// two waves: its chains, each a serial run of about 2,000 jumps per window, need waves in
+// flight), one for G >= 4 (AGPRs as well).
 // EXT: the variant with the less common features, launched when an ensemble of the pass
 // needs one of them (it runs every other ensemble as well): MSD(t) (R1 tracking,
 // centre-of-mass samples), the polydisperse CD background (partner tables) and flow
