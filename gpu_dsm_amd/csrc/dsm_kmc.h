@@ -1078,7 +1078,8 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
     int pidx = 0;
     const float* pd = gtabs + E.P->pd_off;
     const bool partner = EXT && E.partner;
-    if (partner) {
+    // (only creations draw a partner; the draw runs when a chain of the wave creates)
+    if (partner && __ballot(kind == kEvCDc || kind == kEvCDcTail || kind == kEvSDcHead || kind == kEvSDcTail)) {
       const uint64_t evj = H.ev - 1;
       const uint4 x = philox4x32_10(make_uint4((uint32_t)evj, (uint32_t)(evj >> 32), H.cid, kPurposeStep), E.key0,
                                     E.key1);
