@@ -41,10 +41,10 @@ CANDIDATES = {
     "lds_table_floats": [2816, 0, 8192],
     "blocks_per_cu": [0, 1, 2],
     "window": [256, 16, 64],
-    "wide_pair": [0, 1],
+    "wide_pair": [1, 0],
 }
 DEFAULTS = {"small_per_cu": 12, "concurrent": -1, "split_margin": 16, "lds_table_floats": 2816, "blocks_per_cu": 0,
-            "window": 256, "wide_pair": 0}
+            "window": 256, "wide_pair": 1}
 
 
 def signature(descs):

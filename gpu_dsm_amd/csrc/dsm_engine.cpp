@@ -333,7 +333,7 @@ struct EngineOptions {
   int blocks_per_cu = 0;                 // DSM_OPT_BLOCKS_PER_CU
   int window = kDefaultWindow;           // DSM_OPT_WINDOW
   int graph = 0;                         // DSM_OPT_GRAPH
-  int wide_pair = 0;                     // DSM_OPT_WIDE_PAIR
+  int wide_pair = 1;                     // DSM_OPT_WIDE_PAIR
 };
 
 struct dsm_engine {
@@ -773,9 +773,10 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // end at 21 ms, but the bulk then runs at a lower occupancy meanwhile: 114 ms.
   auto stream = [&](int k) { return k == 0 ? e->st : e->xs[k]; };  // xs[0] is not a stream
   // bin 2 (128 <= Z < 240 at the split) runs two chains per wave in the two-group pair
-  // form when DSM_OPT_WIDE_PAIR is set (a third fewer VALU instructions per jump, the
-  // same window time on the 262,144-chain melt: 129.1 against 129.8 ms); the late round
-  // keeps one chain per wave
+  // form unless DSM_OPT_WIDE_PAIR is 0 (a third fewer VALU instructions per jump; on the
+  // 262,144-chain melt 93.6-94.2 ms per window against 95.0-96.7 ms in the G = 1 wave
+  // form, since the strand lengths left the registers and with them most of its spills);
+  // the late round keeps one chain per wave
   auto first = [&](int b, int k) -> hipError_t {
     if (b > ps.bin_max) return hipSuccess;
     bind(kSplitList + b, late_of(b));

@@ -45,7 +45,7 @@ SETTINGS = [
     {},
     dict(concurrent=0),
     dict(concurrent=1, split_margin=0),
-    dict(concurrent=1, wide_pair=1),
+    dict(concurrent=1, wide_pair=0),
     dict(concurrent=1, split_margin=4, blocks_per_cu=2),
     dict(small_per_cu=0, lds_table_floats=0),
     dict(small_per_cu=100000),
