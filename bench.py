@@ -7,7 +7,7 @@ chains, Nk = round(LogNormal) with number average 200 and Mw/Mn = 1.5 clipped to
 length distribution (weight-averaged), G(t) sampled every tau_K into the multi-tau
 correlator (16 points per level, block factor 2).  One "step" advances every chain by
 --window Kuhn times through the batched persistent form (chains split by Z into six
-geometries run side by side on four streams), plus the correlator update.  All inputs
+bins whose kernels run side by side on four streams), plus the correlator update.  All inputs
 are synthetic (Philox-keyed equilibrium draw); the state is resident in HBM when the
 timed region starts.
 
@@ -416,9 +416,10 @@ def main():
         "config": {"workload": "polydisperse melt, 262144 chains, Nk log-normal (Mn 200, PDI 1.5) clipped to "
                                "[10, 2000], beta=1, CD on with melt partners, G(t) multi-tau p=16 m=2, "
                                "%d tau_K per step" % args.window,
-                   "form": "batched persistent: chains split by Z into six geometries (row form 4 chains per "
-                           "wave Z<=64, pair form 2 per wave Z<=128, wave form 1 per wave Z<=256/512/1024/2048) "
-                           "run side by side on four streams, then a late round for chains that outgrew theirs",
+                   "form": "batched persistent: chains split by Z into six bins (row form 4 chains per wave "
+                           "Z<64, pair form 2 per wave Z<128, two-group pair form 2 per wave Z<240, wave form 1 "
+                           "per wave Z<480/960/2048) run side by side on four streams, then a late round for "
+                           "chains that outgrew theirs",
                    "n_chains": C3["n_chains"], "nk_max": int(c3["nk"].max()), "nk_mean": float(c3["nk"].mean()),
                    "window_tau_k": args.window},
         "roofline": rl,
