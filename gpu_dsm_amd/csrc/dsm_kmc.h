@@ -140,7 +140,7 @@ struct Geo {
 };
 
 // LDS staging arrays of a wave (each STG floats; chain positions at srow + p)
-enum StageArray : int { kStN = 0, kStQ2, kStW, kStWP, kStWM, kStRR, kStG, kStArrays };
+enum StageArray : int { kStN = 0, kStQ2, kStW, kStWP, kStWM, kStRR, kStArrays };
 
 template <int CPW>
 struct LaneC {
@@ -953,8 +953,6 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   // 2. scan, total, clock
   float Gs[4 * G];
   const float W = chain_scan<CPW, G>(v, Gs, Z, L);
-#pragma unroll
-  for (int g = 0; g < G; ++g) S.put4(kStG, g, L.pos0, Gs[4 * g], Gs[4 * g + 1], Gs[4 * g + 2], Gs[4 * g + 3]);
   const float u_sel = seg_get<CPW>(RC.sel, k_rng, L);
   float dt;
   if (E.exp_wait) {
@@ -965,7 +963,6 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   }
   const float pr = W * u_sel;
   const int p = pick_position<CPW, G>(Gs, pr, Z, L);
-  wave_lds_sync();
   // 3. sub-event (per lane, replicated per chain); the pair and wave forms compute the
   // chain-end rates one by one only where a chain picked its end position
   EndRates er = er_row;
@@ -974,7 +971,16 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   }
   const float cdcz = er.cdcz, sdc1 = er.sdc1, sdcz = er.sdcz, sdd1 = er.sdd1, sddz = er.sddz;
   const int pa = p < CAP - 2 ? p : CAP - 2;  // in-range position for the reads below
-  const float Gprev = S.getf(kStG, pa > 0 ? pa - 1 : 0);
+  // the inclusive sum before pa: its lane picks the slot, one cross-lane read
+  float Gprev;
+  {
+    const int q = pa > 0 ? pa - 1 : 0;
+    const int jq = 4 * (q / Geo<CPW, G>::GS) + (q & 3);
+    float mine = Gs[0];
+#pragma unroll
+    for (int k = 1; k < 4 * G; ++k) mine = (jq == k) ? Gs[k] : mine;
+    Gprev = seg_get<CPW>(mine, (q >> 2) & (SEG - 1), L);
+  }
   const int Np = S.geti(kStN, pa), Np1 = S.geti(kStN, pa + 1);
   const float wpp = S.getf(kStWP, pa), wmp = S.getf(kStWM, pa);
   const float wcp = S.getf(kStW, pa), wcp1 = S.getf(kStW, pa + 1);
