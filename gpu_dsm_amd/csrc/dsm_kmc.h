@@ -141,6 +141,14 @@ struct Geo {
 
 // LDS staging arrays of a wave (each STG floats; chain positions at srow + p)
 enum StageArray : int { kStN = 0, kStQ2, kStW, kStWP, kStWM, kStRR, kStArrays };
+// The row form also stages the scan's inclusive sums (array kStG): its sub-event reads
+// G[p-1] there, cheaper for four chains per wave than the pair and wave forms' slot
+// select and cross-lane read, and its seven arrays still leave LDS for three waves per
+// SIMD (the G = 2 and two-group pair kernels, at 512 floats per array, would get 11
+// waves per CU instead of 12 with a seventh).
+constexpr int kStG = kStArrays;
+template <int CPW>
+constexpr bool kStageG = CPW == 4;
 
 template <int CPW>
 struct LaneC {
@@ -953,6 +961,10 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   // 2. scan, total, clock
   float Gs[4 * G];
   const float W = chain_scan<CPW, G>(v, Gs, Z, L);
+  if constexpr (kStageG<CPW>) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) S.put4(kStG, g, L.pos0, Gs[4 * g], Gs[4 * g + 1], Gs[4 * g + 2], Gs[4 * g + 3]);
+  }
   const float u_sel = seg_get<CPW>(RC.sel, k_rng, L);
   float dt;
   if (E.exp_wait) {
@@ -963,6 +975,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   }
   const float pr = W * u_sel;
   const int p = pick_position<CPW, G>(Gs, pr, Z, L);
+  if constexpr (kStageG<CPW>) wave_lds_sync();
   // 3. sub-event (per lane, replicated per chain); the pair and wave forms compute the
   // chain-end rates one by one only where a chain picked its end position
   EndRates er = er_row;
@@ -971,9 +984,12 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   }
   const float cdcz = er.cdcz, sdc1 = er.sdc1, sdcz = er.sdcz, sdd1 = er.sdd1, sddz = er.sddz;
   const int pa = p < CAP - 2 ? p : CAP - 2;  // in-range position for the reads below
-  // the inclusive sum before pa: its lane picks the slot, one cross-lane read
+  // the inclusive sum before pa: staged (row form), or its lane picks the slot and one
+  // cross-lane read fetches it
   float Gprev;
-  {
+  if constexpr (kStageG<CPW>) {
+    Gprev = S.getf(kStG, pa > 0 ? pa - 1 : 0);
+  } else {
     const int q = pa > 0 ? pa - 1 : 0;
     const int jq = 4 * (q / Geo<CPW, G>::GS) + (q & 3);
     float mine = Gs[0];
@@ -1540,7 +1556,7 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
 
 // LDS floats per wave: staging arrays, then the table copy
 template <int CPW, int G>
-__host__ __device__ constexpr int stage_floats() { return kStArrays * Geo<CPW, G>::STG; }
+__host__ __device__ constexpr int stage_floats() { return (kStArrays + (kStageG<CPW> ? 1 : 0)) * Geo<CPW, G>::STG; }
 
 // Persistent kernel: each wave takes work units from the queue until it is empty.
 // PLAIN: a.plain_jumps jumps per chain, the state in HBM between launches (the
@@ -1593,7 +1609,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW ==
 template <bool LT, bool EXT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) kmc_rowpair_kernel(KmcArgs row,
                                                                                                  KmcArgs pair) {
-  static_assert(stage_floats<4, 1>() == stage_floats<2, 1>(), "row and pair forms stage the same floats");
+  static_assert(stage_floats<4, 1>() >= stage_floats<2, 1>(), "the pair form's staging fits the row form's");
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
   lds_f* ltab = stg + stage_floats<4, 1>();
