@@ -776,12 +776,17 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // form unless DSM_OPT_WIDE_PAIR is 0 (a third fewer VALU instructions per jump; on the
   // 262,144-chain melt 93.6-94.2 ms per window against 95.0-96.7 ms in the G = 1 wave
   // form, since the strand lengths left the registers and with them most of its spills);
-  // the late round keeps one chain per wave
+  // the late round keeps one chain per wave.  Only one-ensemble passes: a wave of two
+  // chains runs one ensemble at a time, and in a serving mix of many ensembles its
+  // segments would wait for each other (the C5 serving stream at horizon 60: 6.2e8
+  // jumps/s with the two-chain form, 8.4e8 with one chain per wave; at horizon 40 the
+  // p50 latency 5.6 s against 0.38 s)
+  const bool wide = e->opt.wide_pair && ps.n_ens == 1;
   auto first = [&](int b, int k) -> hipError_t {
     if (b > ps.bin_max) return hipSuccess;
     bind(kSplitList + b, late_of(b));
     ++rs->kmc_launches;
-    const int geom = (b == 2 && e->opt.wide_pair) ? kGeomPair2 : b;
+    const int geom = (b == 2 && wide) ? kGeomPair2 : b;
     return launch_kmc(a, geom, false, grid(geom), stream(k));
   };
   HIPCHK(first(5, 0));

@@ -106,8 +106,8 @@ int dsm_mem_info(int32_t device_id, size_t* free_bytes, size_t* total_bytes);
 #define DSM_OPT_WINDOW 7            /* samples per staging window of ensembles added later (256) */
 #define DSM_OPT_GRAPH 8             /* 1: batched windows run as captured HIP graphs, replayed with the
                                        window's parameters in a pinned block (0) */
-#define DSM_OPT_WIDE_PAIR 9         /* concurrent pass: chains with 128 <= Z < 240 run two per wave (1) or
-                                       one per wave (0) (1) */
+#define DSM_OPT_WIDE_PAIR 9         /* concurrent pass of one ensemble: chains with 128 <= Z < 240 run two
+                                       per wave (1) or one per wave (0) (1) */
 int dsm_set_option(dsm_engine* e, int32_t key, int32_t value);
 int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value);
 
