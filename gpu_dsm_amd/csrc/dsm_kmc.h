@@ -715,8 +715,9 @@ __device__ __forceinline__ void refresh_all(const Stage<CPW, G>& S, const Strand
 // instruction stream: lane j (0..2) evaluates W+ of pair lo+j and lane 8+j its W-
 // (the same FP operations as pair_rate, with the tables and signs chosen per lane);
 // lane j then adds the halves (W- arrives by DPP row_ror:8) and writes W+, W-, R.
-// Lane 3 zeroes the rate cache at zero_at (>= 0): the old end position, past the end
-// after a destruction.
+// Lane 3 zeroes the rate cache at zero_at (>= 0) after a destruction: the old end
+// position (pair and wave forms, whose cache holds the chain-end rates there) or the new
+// one (row form).
 template <int CPW, int G, bool LT>
 __device__ __forceinline__ void refresh_pairs(const Stage<CPW, G>& S, const Tabs<LT>& T, int cd, int Z, int lo, int hi,
                                               int zero_at, const LaneC<CPW>& L) {
@@ -1562,7 +1563,7 @@ __host__ __device__ constexpr int stage_floats() { return (kStArrays + (kStageG<
 // PLAIN: a.plain_jumps jumps per chain, the state in HBM between launches (the
 // per-launch form; one jump per launch is the reference's structure); otherwise run to
 // the targets.  Both forms are held to the same occupancy targets: three waves per SIMD
-// (168 VGPRs) up to G = 2 (the G = 2 wave form spills 20 bytes per lane there; with 36
+// (168 VGPRs) up to G = 2 (the G = 2 wave form spills 12 bytes per lane there; with 36
 // spilled it finished the 262,144-chain melt's window in 117-126 ms against 130-131 ms at
 // two waves: its chains, each a serial run of about 2,000 jumps per window, need waves in
 // flight), one for G >= 4 (AGPRs as well).
