@@ -129,7 +129,7 @@ __device__ __forceinline__ int split_newn(float r, int N, float wc) {
 // ------------------------------------------------------------------ geometry
 template <int CPW, int G>
 struct Geo {
-  static_assert((CPW == 4 && G == 1) || (CPW == 2 && (G == 1 || G == 2)) ||
+  static_assert((CPW == 4 && (G == 1 || G == 2)) || (CPW == 2 && (G == 1 || G == 2)) ||
                     (CPW == 1 && (G == 1 || G == 2 || G == 4 || G == 8)),
                 "unsupported chain geometry");
   static constexpr int R = 4 * G;                        // registers per strand array
@@ -147,8 +147,8 @@ enum StageArray : int { kStN = 0, kStQ2, kStW, kStWP, kStWM, kStRR, kStArrays };
 // SIMD (the G = 2 and two-group pair kernels, at 512 floats per array, would get 11
 // waves per CU instead of 12 with a seventh).
 constexpr int kStG = kStArrays;
-template <int CPW>
-constexpr bool kStageG = CPW == 4;
+template <int CPW, int G>
+constexpr bool kStageG = CPW == 4 && G == 1;
 
 template <int CPW>
 struct LaneC {
@@ -310,7 +310,17 @@ __device__ __forceinline__ void block_scan4(float v0, float v1, float v2, float 
 template <int CPW, int G>
 __device__ __forceinline__ float chain_scan(const float (&v)[4 * G], float (&out)[4 * G], int Z,
                                             const LaneC<CPW>& L) {
-  if constexpr (CPW == 4) {
+  if constexpr (CPW == 4 && G == 2) {
+    // blocks 0 and 1 of the chain are its row in register groups 0 and 1; block 1 adds
+    // block 0's total
+    float I1[4];
+    block_scan4(v[0], v[1], v[2], v[3], out);
+    block_scan4(v[4], v[5], v[6], v[7], I1);
+    const float B0 = row_last(out[3]), B1 = row_last(I1[3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[4 + k] = B0 + I1[k];
+    return Z > 64 ? B0 + B1 : B0;
+  } else if constexpr (CPW == 4) {
     block_scan4(v[0], v[1], v[2], v[3], out);
     return row_last(out[3]);
   } else if constexpr (CPW == 2 && G >= 2) {
@@ -380,7 +390,25 @@ __device__ __forceinline__ float chain_total(const float (&v)[4 * G], int Z, con
 template <int CPW, int G>
 __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr, int Z, const LaneC<CPW>& L) {
   constexpr int CAP = Geo<CPW, G>::CAP;
-  if constexpr (CPW == 4) {
+  if constexpr (CPW == 4 && G == 2) {
+    // per group: each lane's first slot, then the row minimum; group 1 holds the later
+    // positions, so the minimum over the groups is the first one
+    constexpr int kIdent = 0x7fffffff;
+    int res = CAP;
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int b0 = 64 * g + L.pos0;
+      int cand = CAP;
+#pragma unroll
+      for (int k = 3; k >= 0; --k) cand = (Gv[4 * g + k] > pr && b0 + k < Z) ? b0 + k : cand;
+      cand = min(cand, dppi<kRowShr1>(cand, kIdent));
+      cand = min(cand, dppi<kRowShr2>(cand, kIdent));
+      cand = min(cand, dppi<kRowShr4>(cand, kIdent));
+      cand = min(cand, dppi<kRowShr8>(cand, kIdent));
+      res = min(res, row_last(cand));
+    }
+    return res;
+  } else if constexpr (CPW == 4) {
     int cand = CAP;
 #pragma unroll
     for (int k = 3; k >= 0; --k) cand = (Gv[k] > pr && L.pos0 + k < Z) ? L.pos0 + k : cand;
@@ -962,7 +990,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   // 2. scan, total, clock
   float Gs[4 * G];
   const float W = chain_scan<CPW, G>(v, Gs, Z, L);
-  if constexpr (kStageG<CPW>) {
+  if constexpr (kStageG<CPW, G>) {
 #pragma unroll
     for (int g = 0; g < G; ++g) S.put4(kStG, g, L.pos0, Gs[4 * g], Gs[4 * g + 1], Gs[4 * g + 2], Gs[4 * g + 3]);
   }
@@ -976,7 +1004,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   }
   const float pr = W * u_sel;
   const int p = pick_position<CPW, G>(Gs, pr, Z, L);
-  if constexpr (kStageG<CPW>) wave_lds_sync();
+  if constexpr (kStageG<CPW, G>) wave_lds_sync();
   // 3. sub-event (per lane, replicated per chain); the pair and wave forms compute the
   // chain-end rates one by one only where a chain picked its end position
   EndRates er = er_row;
@@ -988,7 +1016,7 @@ __device__ __forceinline__ void jump(Strands<G>& C, Hdr& H, bool act, const Stag
   // the inclusive sum before pa: staged (row form), or its lane picks the slot and one
   // cross-lane read fetches it
   float Gprev;
-  if constexpr (kStageG<CPW>) {
+  if constexpr (kStageG<CPW, G>) {
     Gprev = S.getf(kStG, pa > 0 ? pa - 1 : 0);
   } else {
     const int q = pa > 0 ? pa - 1 : 0;
@@ -1557,7 +1585,10 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
 
 // LDS floats per wave: staging arrays, then the table copy
 template <int CPW, int G>
-__host__ __device__ constexpr int stage_floats() { return (kStArrays + (kStageG<CPW> ? 1 : 0)) * Geo<CPW, G>::STG; }
+__host__ __device__ constexpr int stage_floats() { return (kStArrays + (kStageG<CPW, G> ? 1 : 0)) * Geo<CPW, G>::STG; }
+// the merged row-form launch: four chains per wave with one register group, then with two
+constexpr int kRowPairStage =
+    stage_floats<4, 1>() > stage_floats<4, 2>() ? stage_floats<4, 1>() : stage_floats<4, 2>();
 
 // Persistent kernel: each wave takes work units from the queue until it is empty.
 // PLAIN: a.plain_jumps jumps per chain, the state in HBM between launches (the
@@ -1610,12 +1641,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW ==
 template <bool LT, bool EXT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) kmc_rowpair_kernel(KmcArgs row,
                                                                                                  KmcArgs pair) {
-  static_assert(stage_floats<4, 1>() >= stage_floats<2, 1>(), "the pair form's staging fits the row form's");
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
-  lds_f* ltab = stg + stage_floats<4, 1>();
+  lds_f* ltab = stg + kRowPairStage;
   run_stream<4, 1, LT, EXT>(row, stg, ltab);
-  run_stream<2, 1, LT, EXT>(pair, stg, ltab);
+  run_stream<4, 2, LT, EXT>(pair, stg, ltab);
 }
 
 }  // namespace dsm

@@ -21,11 +21,11 @@ namespace dsm {
 #if DSM_GEOM == 6
 namespace {
 constexpr int kMaxLdsFloats = 65536 / 4;
-bool tables_fit(int lds_floats) { return lds_floats > 0 && stage_floats<4, 1>() + lds_floats <= kMaxLdsFloats; }
+bool tables_fit(int lds_floats) { return lds_floats > 0 && kRowPairStage + lds_floats <= kMaxLdsFloats; }
 
 template <bool LT, bool EXT>
 hipError_t launch_rp(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st) {
-  const size_t lds = sizeof(float) * ((size_t)stage_floats<4, 1>() + (LT ? (size_t)row.lds_floats : 0));
+  const size_t lds = sizeof(float) * ((size_t)kRowPairStage + (LT ? (size_t)row.lds_floats : 0));
   hipLaunchKernelGGL((kmc_rowpair_kernel<LT, EXT>), dim3(grid), dim3(64), lds, st, row, pair);
   return hipGetLastError();
 }
@@ -33,7 +33,7 @@ hipError_t launch_rp(const KmcArgs& row, const KmcArgs& pair, int grid, hipStrea
 template <bool LT, bool EXT>
 int occupancy_rp(int lds_floats) {
   int nb = 0;
-  const size_t lds = sizeof(float) * ((size_t)stage_floats<4, 1>() + (LT ? (size_t)lds_floats : 0));
+  const size_t lds = sizeof(float) * ((size_t)kRowPairStage + (LT ? (size_t)lds_floats : 0));
   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_rowpair_kernel<LT, EXT>, 64, lds);
   return e == hipSuccess ? nb : 0;
 }
