@@ -12,7 +12,8 @@ common/ensemble_kernel.cu:41-42).  Here the per-pass layout is chosen per worklo
   blocks_per_cu     persistent waves per CU (occupancy below the register limit);
   window            samples per staging window: the stress / correlator sync interval;
   wide_pair         chains with 128 <= Z < 240 two per wave (two register groups each)
-                    instead of one per wave, in concurrent passes.
+                    instead of one per wave, and those with 64 <= Z < 128 four per wave
+                    instead of two, in concurrent passes of one ensemble.
 
 No option changes a result (every chain makes the same jumps with the same Philox
 numbers under any setting; tests/test_gpu_options.py), so tuning is free to pick on

@@ -732,8 +732,9 @@ int late_last(int bin_max) { return std::max(bin_max, kLateFirst); }
 // the device), so the latency-bound runs of long chains overlap the bulk instead of
 // following it.  Chains that outgrow their geometry during this round go to late lists
 // that a second round runs through the larger geometries (the first of them behind the
-// row and pair forms, the others in order after the whole first round).  Every chain makes
-// the same jumps to the same targets as in the sequential order.
+// merged row-form launch of bins 0 and 1, the others in order after the whole first
+// round).  Every chain makes the same jumps to the same targets as in the sequential
+// order.
 int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   int* cnt = e->d_cnt;
   const size_t cap = (size_t)e->d_work_cap;
@@ -760,7 +761,7 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // waves -- each leaves no room for another wave on its SIMD -- are dispatched before the other
   // queues, released by the split's event, fill the device (launched on a side stream
   // they waited for a free SIMD: 140-180 ms for a 68 ms run).  G = 4 (also latency-bound)
-  // then on the second stream, the G = 1 bulk behind it, the merged row and pair forms on
+  // then on the second stream, the G = 1 bulk behind it, the merged bin-0/bin-1 row forms on
   // the third stream, the G = 2 bulk on the fourth.  The engine stream then runs second
   // launches of the G = 2 and G = 1 bins, which share their work queues, so whichever
   // stream is free first takes the remaining chains (a launch that finds its queue empty
@@ -780,7 +781,10 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   // chains runs one ensemble at a time, and in a serving mix of many ensembles its
   // segments would wait for each other (the C5 serving stream at horizon 60: 6.2e8
   // jumps/s with the two-chain form, 8.4e8 with one chain per wave; at horizon 40 the
-  // p50 latency 5.6 s against 0.38 s)
+  // p50 latency 5.6 s against 0.38 s).  The same switch puts bin 1 (64 <= Z < 128) in the
+  // row form with two register groups, four chains per wave, where the pair form runs two
+  // (one-ensemble 262,144-chain melt: 80-82 ms per window against 89.7-90.3 ms; the C5
+  // stream at horizon 60: 7.9e8 jumps/s against 8.4e8 in the pair form)
   const bool wide = e->opt.wide_pair && ps.n_ens == 1;
   auto first = [&](int b, int k) -> hipError_t {
     if (b > ps.bin_max) return hipSuccess;
@@ -797,12 +801,12 @@ int pass_bins_concurrent(dsm_engine* e, Pass& ps, dsm_run_stats* rs) {
   bind(kSplitList + 1, late_of(1));
   const KmcArgs pair_args = a;
   bind(kSplitList, late_of(0));
-  int nb = kmc_rowpair_occupancy(a.lds_floats, a.ext != 0);
+  int nb = kmc_rowpair_occupancy(a.lds_floats, a.ext != 0, wide);
   if (e->opt.blocks_per_cu > 0) nb = std::min(nb, e->opt.blocks_per_cu);
   if (timed) HIPCHK(hipEventRecord(e->cev[1], e->xs[2]));
-  HIPCHK(launch_kmc_rowpair(a, pair_args, std::max(nb, 1) * e->n_cu, e->xs[2]));
+  HIPCHK(launch_kmc_rowpair(a, pair_args, wide, std::max(nb, 1) * e->n_cu, e->xs[2]));
   if (timed) HIPCHK(hipEventRecord(e->cev[2], e->xs[2]));
-  // the row and pair forms are the only feeders of the first late list, complete when
+  // bins 0 and 1 are the only feeders of the first late list, complete when
   // their launch ends: its run follows on the same stream, beside the other geometries'
   // bulk, instead of after the whole first round (118-123 ms per window against 123-126
   // with the earlier stream layout)
@@ -1889,15 +1893,16 @@ static hipError_t l_philox(const void* i, void* o, int n, int, hipStream_t s) {
 }
 
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form) {
-  if (!in || !out || form < 0 || form > 3 || n <= 0 ||
-      n > (form == 1 ? 1024 : form == 2 ? 128 : form == 3 ? 256 : 64))
+  if (!in || !out || form < 0 || form > 4 || n <= 0 ||
+      n > (form == 1 ? 1024 : form == 2 ? 128 : form == 3 ? 256 : form == 4 ? 128 : 64))
     return fail(DSM_ERR_ARG, "bad probe arguments");
   const int chains = form == 1 ? 1 : (form == 2 || form == 3) ? 2 : 4;
   const int nin = chains * n, nout = chains * (n + 1);
   return probe_run(device_id, in, 4 * (size_t)nin, out, 4 * (size_t)nout, l_scan, n, form);
 }
 int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom) {
-  if (!in || !out || geom < 0 || (geom > 4 && geom != kGeomPair2) || n < 1 || n > kmc_geom_capacity(geom))
+  if (!in || !out || geom < 0 || (geom > 4 && geom != kGeomPair2 && geom != kGeomRow2) || n < 1 ||
+      n > kmc_geom_capacity(geom))
     return fail(DSM_ERR_ARG, "bad probe arguments");
   return probe_run(device_id, in, 4 * (size_t)(4 * n + 1), out, 4 * 3, l_com, n, geom);
 }

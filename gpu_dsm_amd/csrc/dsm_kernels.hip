@@ -645,7 +645,8 @@ __global__ void copy_probe_kernel(const float4* __restrict__ in, float4* __restr
 // (wave form, four register groups): out[0..n) inclusive sums, out[n] total.
 // form 0: four chains of n <= 64 positions (row form), form 2: two chains of n <= 128
 // positions (pair form), form 3: two chains of n <= 256 positions (pair form with two
-// register groups): chain r reads in[r*n ..] and writes out[r*(n+1) ..].
+// register groups), form 4: four chains of n <= 128 positions (row form with two register
+// groups): chain r reads in[r*n ..] and writes out[r*(n+1) ..].
 // Centre-of-mass sample of one chain state in a chosen geometry (0 row form, 1 pair form,
 // 2/3/4 wave form G = 1/2/4, 7 pair form with two groups): in[4i..4i+3] = Qx Qy Qz N of
 // strand i, in[4n] = Nk; out = 3 floats.
@@ -681,6 +682,7 @@ __global__ void __launch_bounds__(64) com_probe_kernel(const float* in, float* o
   else if (geom == 2) com_probe<1, 1>(in, out, n, b);
   else if (geom == 3) com_probe<1, 2>(in, out, n, b);
   else if (geom == kGeomPair2) com_probe<2, 2>(in, out, n, b);
+  else if (geom == kGeomRow2) com_probe<4, 2>(in, out, n, b);
   else com_probe<1, 4>(in, out, n, b);
 }
 
@@ -712,6 +714,22 @@ __global__ void __launch_bounds__(64) scan_probe_kernel(const float* in, float* 
     const float tot = chain_scan<2, 1>(v, g, n, L);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
+      const int p = L.pos(j);
+      if (p < n) out[r * (n + 1) + p] = g[j];
+    }
+    if (L.sl == 0) out[r * (n + 1) + n] = tot;
+  } else if (form == 4) {
+    const LaneC<4> L;
+    const int r = L.lane >> 4;
+    float v[8], g[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = L.pos(j);
+      v[j] = p < n ? in[r * n + p] : 0.0f;
+    }
+    const float tot = chain_scan<4, 2>(v, g, n, L);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
       const int p = L.pos(j);
       if (p < n) out[r * (n + 1) + p] = g[j];
     }

@@ -1586,9 +1586,6 @@ __device__ __forceinline__ void run_stream(const KmcArgs& a, lds_f* stg, lds_f* 
 // LDS floats per wave: staging arrays, then the table copy
 template <int CPW, int G>
 __host__ __device__ constexpr int stage_floats() { return (kStArrays + (kStageG<CPW, G> ? 1 : 0)) * Geo<CPW, G>::STG; }
-// the merged row-form launch: four chains per wave with one register group, then with two
-constexpr int kRowPairStage =
-    stage_floats<4, 1>() > stage_floats<4, 2>() ? stage_floats<4, 1>() : stage_floats<4, 2>();
 
 // Persistent kernel: each wave takes work units from the queue until it is empty.
 // PLAIN: a.plain_jumps jumps per chain, the state in HBM between launches (the
@@ -1630,22 +1627,35 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((CPW ==
   }
 }
 
-// Row and pair forms in one launch (concurrent passes): each wave takes chains from the
-// row-form list until it is exhausted, then from the pair-form list.  Both shapes run
-// within the same register budget (three waves per SIMD) and the same staging size, so
-// the merged launch costs neither occupancy nor LDS, and it leaves the pass's other
-// hardware queues to the wave-form geometries: the long-chain runs, which set a window's
-// critical path, and the wave-form bulk then all start at once.  Each wave restages and
-// reloads its rate tables when it moves to the pair list, so every chain runs exactly
-// as in the separate launches.
-template <bool LT, bool EXT>
+// Bins 0 and 1 of a concurrent pass in one launch: each wave takes chains from the bin-0
+// list (Z < 64) in the row form until it is exhausted, then from the bin-1 list
+// (64 <= Z < 128).  ROW2 (one-ensemble passes): bin 1 in the row form with two register
+// groups, four chains per wave (the 262,144-chain melt's window: 80-82 ms against
+// 89.7-90.3 ms in the pair form); otherwise in the pair form, two chains per wave (a
+// wave runs one ensemble at a time, and in a serving mix four segments wait for each
+// other more than two: the C5 stream at horizon 60 ran at 7.9e8 jumps/s with the
+// two-group row form, 8.4e8 with the pair form).  Both shapes of a launch run within
+// the same register budget (three waves per SIMD; the two-group row shape spills about
+// 116 bytes per lane) and the larger staging size (12.3 KB with it, within the LDS for
+// three waves per SIMD), and the launch leaves
+// the pass's other hardware queues to the wave-form geometries: the long-chain runs,
+// which set a window's critical path, and the wave-form bulk then all start at once.
+// Each wave restages and reloads its rate tables when it moves to the bin-1 list, so
+// every chain runs exactly as in separate launches.
+template <bool ROW2>
+constexpr int row_pair_stage() {
+  return ROW2 ? (stage_floats<4, 1>() > stage_floats<4, 2>() ? stage_floats<4, 1>() : stage_floats<4, 2>())
+              : (stage_floats<4, 1>() > stage_floats<2, 1>() ? stage_floats<4, 1>() : stage_floats<2, 1>());
+}
+template <bool LT, bool EXT, bool ROW2>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) kmc_rowpair_kernel(KmcArgs row,
                                                                                                  KmcArgs pair) {
   extern __shared__ float lds_dyn[];
   lds_f* stg = (lds_f*)lds_dyn;
-  lds_f* ltab = stg + kRowPairStage;
+  lds_f* ltab = stg + row_pair_stage<ROW2>();
   run_stream<4, 1, LT, EXT>(row, stg, ltab);
-  run_stream<4, 2, LT, EXT>(pair, stg, ltab);
+  if constexpr (ROW2) run_stream<4, 2, LT, EXT>(pair, stg, ltab);
+  else run_stream<2, 1, LT, EXT>(pair, stg, ltab);
 }
 
 }  // namespace dsm

@@ -4,7 +4,8 @@
 //   1: pair form, two chains per wave, Z <= 128
 //   2, 3, 4, 5: wave form, one chain per wave, Z <= 256, 512, 1024, 2048 (the last keeps
 //   part of its state in scratch memory; it carries the rare chains with Z > 1024)
-//   6: the row and pair forms in one launch (batched, concurrent passes)
+//   6: bins 0 and 1 of a concurrent pass in one launch: the row form with one register
+//      group (Z < 64), then with two (64 <= Z < 128)
 //   7: pair form with two register groups: two chains per wave, Z <= 256 (the bulk of
 //      128 <= Z < 240 in concurrent passes)
 #include "dsm_kmc.h"
@@ -21,35 +22,47 @@ namespace dsm {
 #if DSM_GEOM == 6
 namespace {
 constexpr int kMaxLdsFloats = 65536 / 4;
-bool tables_fit(int lds_floats) { return lds_floats > 0 && kRowPairStage + lds_floats <= kMaxLdsFloats; }
+template <bool ROW2>
+bool tables_fit(int lds_floats) { return lds_floats > 0 && row_pair_stage<ROW2>() + lds_floats <= kMaxLdsFloats; }
 
-template <bool LT, bool EXT>
+template <bool LT, bool EXT, bool ROW2>
 hipError_t launch_rp(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st) {
-  const size_t lds = sizeof(float) * ((size_t)kRowPairStage + (LT ? (size_t)row.lds_floats : 0));
-  hipLaunchKernelGGL((kmc_rowpair_kernel<LT, EXT>), dim3(grid), dim3(64), lds, st, row, pair);
+  const size_t lds = sizeof(float) * ((size_t)row_pair_stage<ROW2>() + (LT ? (size_t)row.lds_floats : 0));
+  hipLaunchKernelGGL((kmc_rowpair_kernel<LT, EXT, ROW2>), dim3(grid), dim3(64), lds, st, row, pair);
   return hipGetLastError();
 }
 
-template <bool LT, bool EXT>
+template <bool LT, bool EXT, bool ROW2>
 int occupancy_rp(int lds_floats) {
   int nb = 0;
-  const size_t lds = sizeof(float) * ((size_t)kRowPairStage + (LT ? (size_t)lds_floats : 0));
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_rowpair_kernel<LT, EXT>, 64, lds);
+  const size_t lds = sizeof(float) * ((size_t)row_pair_stage<ROW2>() + (LT ? (size_t)lds_floats : 0));
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kmc_rowpair_kernel<LT, EXT, ROW2>, 64, lds);
   return e == hipSuccess ? nb : 0;
+}
+
+template <bool ROW2>
+hipError_t launch_rp_any(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st) {
+  const bool lt = tables_fit<ROW2>(row.lds_floats);
+  if (row.ext)
+    return lt ? launch_rp<true, true, ROW2>(row, pair, grid, st) : launch_rp<false, true, ROW2>(row, pair, grid, st);
+  return lt ? launch_rp<true, false, ROW2>(row, pair, grid, st) : launch_rp<false, false, ROW2>(row, pair, grid, st);
+}
+
+template <bool ROW2>
+int occupancy_rp_any(int lds_floats, bool ext) {
+  const bool lt = tables_fit<ROW2>(lds_floats);
+  if (ext) return lt ? occupancy_rp<true, true, ROW2>(lds_floats) : occupancy_rp<false, true, ROW2>(0);
+  return lt ? occupancy_rp<true, false, ROW2>(lds_floats) : occupancy_rp<false, false, ROW2>(0);
 }
 }  // namespace
 
-hipError_t launch_kmc_rowpair(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st) {
+hipError_t launch_kmc_rowpair(const KmcArgs& row, const KmcArgs& pair, bool row2, int grid, hipStream_t st) {
   if (grid <= 0) return hipSuccess;
-  const bool lt = tables_fit(row.lds_floats);
-  if (row.ext) return lt ? launch_rp<true, true>(row, pair, grid, st) : launch_rp<false, true>(row, pair, grid, st);
-  return lt ? launch_rp<true, false>(row, pair, grid, st) : launch_rp<false, false>(row, pair, grid, st);
+  return row2 ? launch_rp_any<true>(row, pair, grid, st) : launch_rp_any<false>(row, pair, grid, st);
 }
 
-int kmc_rowpair_occupancy(int lds_floats, bool ext) {
-  const bool lt = tables_fit(lds_floats);
-  if (ext) return lt ? occupancy_rp<true, true>(lds_floats) : occupancy_rp<false, true>(0);
-  return lt ? occupancy_rp<true, false>(lds_floats) : occupancy_rp<false, false>(0);
+int kmc_rowpair_occupancy(int lds_floats, bool ext, bool row2) {
+  return row2 ? occupancy_rp_any<true>(lds_floats, ext) : occupancy_rp_any<false>(lds_floats, ext);
 }
 
 #else

@@ -17,9 +17,14 @@ constexpr int kNumGeoms = 6;
 hipError_t launch_kmc(const KmcArgs& a, int geom, bool plain, int grid, hipStream_t st);
 int kmc_max_blocks_per_cu(int geom, bool plain, int lds_floats, bool ext);
 // Kernel geometries 0..5 hold 64 << g positions per chain (the work bins of a pass);
-// geometry 7 is the two-group pair form, a second kernel for bin 2 (256 positions).
+// geometry 7 is the two-group pair form, a second kernel for bin 2 (256 positions);
+// shape 8, the row form with two register groups (128 positions, bin 1 of the merged
+// row-form launch), names that shape for the probes only.
 constexpr int kGeomPair2 = 7;
-__host__ __device__ constexpr int kmc_geom_capacity(int geom) { return geom == kGeomPair2 ? 256 : 64 << geom; }
+constexpr int kGeomRow2 = 8;
+__host__ __device__ constexpr int kmc_geom_capacity(int geom) {
+  return geom == kGeomPair2 ? 256 : geom == kGeomRow2 ? 128 : 64 << geom;
+}
 #define DSM_DECLARE_GEOM(G)                                                                   \
   hipError_t launch_kmc_geom_##G(const KmcArgs& a, bool plain, int grid, hipStream_t st); \
   int kmc_occupancy_geom_##G(bool plain, int lds_floats, bool ext);
@@ -31,9 +36,10 @@ DSM_DECLARE_GEOM(4)
 DSM_DECLARE_GEOM(5)
 DSM_DECLARE_GEOM(7)
 #undef DSM_DECLARE_GEOM
-// the row form (list `row`) and the pair form (list `pair`) in one batched launch
-hipError_t launch_kmc_rowpair(const KmcArgs& row, const KmcArgs& pair, int grid, hipStream_t st);
-int kmc_rowpair_occupancy(int lds_floats, bool ext);
+// the row form (list `row`) and bin 1 (list `pair`: the row form with two register groups
+// if row2, else the pair form) in one batched launch
+hipError_t launch_kmc_rowpair(const KmcArgs& row, const KmcArgs& pair, bool row2, int grid, hipStream_t st);
+int kmc_rowpair_occupancy(int lds_floats, bool ext, bool row2);
 hipError_t launch_init_eq(const ChainArrays& ch, const EnsParams* ens, const float* tabs, const int* chains, int n,
                           hipStream_t st);
 hipError_t launch_fill_u32(void* p, uint32_t v, int64_t n, hipStream_t st);

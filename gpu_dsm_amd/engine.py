@@ -397,7 +397,7 @@ class Engine:
 def probe_com(Q, N, geom, device=0):
     """MSD centre-of-mass sample (relative to R1) of one chain state computed by the
     KMC kernels' code in geometry `geom` (0 row form, 1 pair form, 2/3/4 wave form
-    G = 1/2/4, 7 pair form with two register groups)."""
+    G = 1/2/4, 7 pair form with two register groups, 8 row form with two register groups)."""
     Q = np.asarray(Q, np.float32).reshape(-1, 3)
     N = np.asarray(N)
     n = Q.shape[0]
@@ -413,14 +413,15 @@ def probe_scan(values, form=1, device=0):
     """Canonical scan tree on the device.  form 1: one chain (len <= 1024) -> (inclusive
     sums, total).  form 0: values shaped (4, n<=64), four row-form chains -> (sums (4, n),
     totals (4,)); form 2: values shaped (2, n<=128), two pair-form chains; form 3: (2, n<=256),
-    two chains of the pair form with two register groups."""
+    two chains of the pair form with two register groups; form 4: (4, n<=128), four chains of
+    the row form with two register groups."""
     v = np.ascontiguousarray(values, np.float32)
     if form == 1:
         n = v.size
         out = np.zeros(n + 1, np.float32)
         _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, 1))
         return out[:n], out[n]
-    rows = 2 if form in (2, 3) else 4
+    rows = 2 if form in (2, 3) else 4  # forms 0 and 4: four chains
     v = v.reshape(rows, -1)
     n = v.shape[1]
     out = np.zeros((rows, n + 1), np.float32)

@@ -107,7 +107,8 @@ int dsm_mem_info(int32_t device_id, size_t* free_bytes, size_t* total_bytes);
 #define DSM_OPT_GRAPH 8             /* 1: batched windows run as captured HIP graphs, replayed with the
                                        window's parameters in a pinned block (0) */
 #define DSM_OPT_WIDE_PAIR 9         /* concurrent pass of one ensemble: chains with 128 <= Z < 240 run two
-                                       per wave (1) or one per wave (0) (1) */
+                                       per wave and those with 64 <= Z < 128 four (1), or one and two (0)
+                                       (1) */
 int dsm_set_option(dsm_engine* e, int32_t key, int32_t value);
 int dsm_get_option(dsm_engine* e, int32_t key, int32_t* value);
 
@@ -231,14 +232,16 @@ int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* out24);
    n <= 1024 positions, out = n inclusive sums + total; form 0: four chains of
    n <= 64 positions in the row form, in = 4*n values, out = 4*(n+1); form 2: two
    chains of n <= 128 positions in the pair form, in = 2*n, out = 2*(n+1); form 3: two
-   chains of n <= 256 positions in the pair form with two register groups),
+   chains of n <= 256 positions in the pair form with two register groups; form 4: four
+   chains of n <= 128 positions in the row form with two register groups),
    exp/log/sin/cos (which 0..3), Philox (6 words in, 4 out). */
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form);
 int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which);
 int dsm_probe_philox(int32_t device_id, const uint32_t* ctr_key, uint32_t* out, int32_t n);
 /* MSD centre-of-mass sample of one chain state (relative to R1) in KMC geometry `geom`
    (0 row form, n <= 64; 1 pair form, n <= 128; 2/3/4 wave form, n <= 256/512/1024;
-   7 pair form with two register groups, n <= 256):
+   7 pair form with two register groups, n <= 256; 8 row form with two register groups,
+   n <= 128):
    in[4i..4i+3] = Qx Qy Qz N
    of strand i, in[4n] = Nk; out = 3 floats. */
 int dsm_probe_com(int32_t device_id, const float* in, float* out, int32_t n, int32_t geom);

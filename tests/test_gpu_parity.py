@@ -81,6 +81,13 @@ def test_scan_tree_matches_oracle(oracle):
         for r in range(2):
             assert np.array_equal(_bits(gpu[r]), _bits(oracle.scan_tree(vals[r])))
             assert _bits(np.float32(tot[r])) == _bits(np.float32(oracle.tree_total(vals[r])))
+    # row form with two register groups: the block carry across the two groups of a row
+    for n in (1, 5, 63, 64, 65, 66, 100, 127, 128):
+        vals = (rng.exponential(size=(4, n)) * 50).astype(np.float32)
+        gpu, tot = g.probe_scan(vals, form=4)
+        for r in range(4):
+            assert np.array_equal(_bits(gpu[r]), _bits(oracle.scan_tree(vals[r])))
+            assert _bits(np.float32(tot[r])) == _bits(np.float32(oracle.tree_total(vals[r])))
 
 
 @pytest.mark.parametrize("which", ["exp", "log", "sin2pi", "cos2pi"])
@@ -238,7 +245,8 @@ def test_gt_samples_and_correlator(engine, oracle):
 @pytest.mark.parametrize("Z,geom", [(1, 0), (2, 0), (60, 0), (64, 0), (1, 1), (63, 1), (65, 1), (100, 1),
                                      (128, 1), (65, 2), (250, 2), (256, 2), (257, 3), (268, 3), (500, 3),
                                      (512, 3), (513, 4), (900, 4), (1024, 4), (1, 7), (100, 7), (128, 7),
-                                     (129, 7), (130, 7), (200, 7), (256, 7)])
+                                     (129, 7), (130, 7), (200, 7), (256, 7), (1, 8), (63, 8), (64, 8),
+                                     (65, 8), (100, 8), (128, 8)])
 def test_com_sample_matches_oracle(oracle, Z, geom):
     """The centre-of-mass sample of the kernels, in every geometry, against the oracle."""
     import gpu_dsm_amd as g
