@@ -417,8 +417,8 @@ def main():
                                "[10, 2000], beta=1, CD on with melt partners, G(t) multi-tau p=16 m=2, "
                                "%d tau_K per step" % args.window,
                    "form": "batched persistent: chains split by Z into six bins (row form 4 chains per wave "
-                           "Z<64, pair form 2 per wave Z<128, two-group pair form 2 per wave Z<240, wave form 1 "
-                           "per wave Z<480/960/2048) run side by side on four streams, then a late round for "
+                           "Z<64, two-group row form 4 per wave Z<128, two-group pair form 2 per wave Z<240, "
+                           "wave form 1 per wave Z<480/960/2048) run side by side on four streams, then a late round for "
                            "chains that outgrew theirs",
                    "n_chains": C3["n_chains"], "nk_max": int(c3["nk"].max()), "nk_mean": float(c3["nk"].mean()),
                    "window_tau_k": args.window},
