@@ -26,8 +26,9 @@ COMMON = [
 ]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["dsm_kernels.hip", "dsm_engine.cpp", "dsm_driver.cpp", "dsm_serve.cpp", "dsm_noalloc.hip"]
-GEOMS = [0, 1, 2, 3, 4, 5, 6, 7]  # KMC chain geometries (one translation unit each; 6 = row and pair merged,
-                                  # 7 = pair form with two register groups)
+GEOMS = [0, 1, 2, 3, 4, 5, 6, 7]  # KMC chain geometries (one translation unit each; 6 = bins 0 and 1 merged:
+                                  # the row form, then the row form with two register groups or the
+                                  # pair form; 7 = pair form with two register groups)
 
 
 def hipcc():
