@@ -352,8 +352,9 @@ def dist_error(g, args, nk_full, n=4096, T=100.0):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
-              "(the path shards by chain ranges across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: the KMC path runs on exactly one GPU; --gpus %d is not supported (across GPUs "
+              "it would shard by chain ranges, one engine per GPU, reducing only stress and correlator sums)"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
     import gpu_dsm_amd as g
 
