@@ -391,23 +391,22 @@ template <int CPW, int G>
 __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr, int Z, const LaneC<CPW>& L) {
   constexpr int CAP = Geo<CPW, G>::CAP;
   if constexpr (CPW == 4 && G == 2) {
-    // per group: each lane's first slot, then the row minimum; group 1 holds the later
-    // positions, so the minimum over the groups is the first one
+    // each lane's first slot over both groups (group 1 holds the later positions, so the
+    // slots are visited from the last of group 1 down to the first of group 0), then one
+    // row minimum: the same position as a minimum per group
     constexpr int kIdent = 0x7fffffff;
-    int res = CAP;
+    int cand = CAP;
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 1; g >= 0; --g) {
       const int b0 = 64 * g + L.pos0;
-      int cand = CAP;
 #pragma unroll
       for (int k = 3; k >= 0; --k) cand = (Gv[4 * g + k] > pr && b0 + k < Z) ? b0 + k : cand;
-      cand = min(cand, dppi<kRowShr1>(cand, kIdent));
-      cand = min(cand, dppi<kRowShr2>(cand, kIdent));
-      cand = min(cand, dppi<kRowShr4>(cand, kIdent));
-      cand = min(cand, dppi<kRowShr8>(cand, kIdent));
-      res = min(res, row_last(cand));
     }
-    return res;
+    cand = min(cand, dppi<kRowShr1>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr2>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr4>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr8>(cand, kIdent));
+    return row_last(cand);
   } else if constexpr (CPW == 4) {
     int cand = CAP;
 #pragma unroll
@@ -420,24 +419,23 @@ __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr,
     cand = min(cand, dppi<kRowShr8>(cand, kIdent));
     return row_last(cand);
   } else if constexpr (CPW == 2 && G >= 2) {
-    // per group: each lane's first slot, the row minimum, then the chain's two rows;
-    // later groups hold later positions, so the minimum over groups is the first one
+    // each lane's first slot over all groups (later groups hold later positions, so the
+    // slots are visited from the last group down to the first), then one minimum over
+    // the chain's two rows: the same position as a minimum per group
     constexpr int kIdent = 0x7fffffff;
-    int res = CAP;
+    int cand = CAP;
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
+    for (int g = G - 1; g >= 0; --g) {
       const int b0 = Geo<CPW, G>::GS * g + L.pos0;
-      int cand = CAP;
 #pragma unroll
       for (int k = 3; k >= 0; --k) cand = (Gv[4 * g + k] > pr && b0 + k < Z) ? b0 + k : cand;
-      cand = min(cand, dppi<kRowShr1>(cand, kIdent));
-      cand = min(cand, dppi<kRowShr2>(cand, kIdent));
-      cand = min(cand, dppi<kRowShr4>(cand, kIdent));
-      cand = min(cand, dppi<kRowShr8>(cand, kIdent));
-      const int r0 = rdl(cand, 15), r1 = rdl(cand, 31), r2 = rdl(cand, 47), r3 = rdl(cand, 63);
-      res = min(res, L.lane >= 32 ? min(r2, r3) : min(r0, r1));
     }
-    return res;
+    cand = min(cand, dppi<kRowShr1>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr2>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr4>(cand, kIdent));
+    cand = min(cand, dppi<kRowShr8>(cand, kIdent));
+    const int r0 = rdl(cand, 15), r1 = rdl(cand, 31), r2 = rdl(cand, 47), r3 = rdl(cand, 63);
+    return L.lane >= 32 ? min(r2, r3) : min(r0, r1);
   } else if constexpr (CPW == 2) {
     int cand = CAP;
 #pragma unroll
