@@ -289,6 +289,17 @@ __device__ __forceinline__ float row_last(float v) {
   return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F0));
 }
 __device__ __forceinline__ int row_last(int v) { return __builtin_amdgcn_ds_swizzle(v, 0x1F0); }
+// lane IDX of each 32-lane half, broadcast over the half (bitmask swizzle and=0 or=IDX):
+// the last lanes of a pair-form chain's two rows without v_readlane and a select
+template <int IDX>
+__device__ __forceinline__ int half_bcast(int v) {
+  static_assert(IDX >= 0 && IDX < 32, "lane within a 32-lane half");
+  return __builtin_amdgcn_ds_swizzle(v, IDX << 5);
+}
+template <int IDX>
+__device__ __forceinline__ float half_bcast(float v) {
+  return __int_as_float(half_bcast<IDX>(__float_as_int(v)));
+}
 
 // canonical block scan of one register group: inclusive values I[0..3]
 __device__ __forceinline__ void block_scan4(float v0, float v1, float v2, float v3, float I[4]) {
@@ -326,15 +337,14 @@ __device__ __forceinline__ float chain_scan(const float (&v)[4 * G], float (&out
   } else if constexpr (CPW == 2 && G >= 2) {
     // chain s holds block 2g in row 2s and block 2g+1 in row 2s+1 of register group g;
     // the carry runs over the blocks in position order, as in the wave form
-    const bool s1 = L.lane >= 32, upper = (L.lane & 16) != 0;
+    const bool upper = (L.lane & 16) != 0;
     const int nb = ((Z - 1) >> 6) + 1;
     float carry = 0.0f, total = 0.0f;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       float I[4];
       block_scan4(v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3], I);
-      const float b00 = rdl(I[3], 15), b01 = rdl(I[3], 31), b10 = rdl(I[3], 47), b11 = rdl(I[3], 63);
-      const float B0 = s1 ? b10 : b00, B1 = s1 ? b11 : b01;
+      const float B0 = half_bcast<15>(I[3]), B1 = half_bcast<31>(I[3]);
       const float c0 = carry;
       carry = (g == 0) ? B0 : carry + B0;
       total = (2 * g == nb - 1) ? carry : total;
@@ -349,9 +359,7 @@ __device__ __forceinline__ float chain_scan(const float (&v)[4 * G], float (&out
     // blocks 0 and 1 of chain s are rows 2s and 2s+1; block 1 adds block 0's total
     float I[4];
     block_scan4(v[0], v[1], v[2], v[3], I);
-    const float b00 = rdl(I[3], 15), b01 = rdl(I[3], 31), b10 = rdl(I[3], 47), b11 = rdl(I[3], 63);
-    const bool s1 = L.lane >= 32;
-    const float B0 = s1 ? b10 : b00, B1 = s1 ? b11 : b01;
+    const float B0 = half_bcast<15>(I[3]), B1 = half_bcast<31>(I[3]);
     const bool upper = (L.lane & 16) != 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) out[k] = upper ? B0 + I[k] : I[k];
@@ -434,8 +442,7 @@ __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr,
     cand = min(cand, dppi<kRowShr2>(cand, kIdent));
     cand = min(cand, dppi<kRowShr4>(cand, kIdent));
     cand = min(cand, dppi<kRowShr8>(cand, kIdent));
-    const int r0 = rdl(cand, 15), r1 = rdl(cand, 31), r2 = rdl(cand, 47), r3 = rdl(cand, 63);
-    return L.lane >= 32 ? min(r2, r3) : min(r0, r1);
+    return min(half_bcast<15>(cand), half_bcast<31>(cand));
   } else if constexpr (CPW == 2) {
     int cand = CAP;
 #pragma unroll
@@ -445,8 +452,7 @@ __device__ __forceinline__ int pick_position(const float (&Gv)[4 * G], float pr,
     cand = min(cand, dppi<kRowShr2>(cand, kIdent));
     cand = min(cand, dppi<kRowShr4>(cand, kIdent));
     cand = min(cand, dppi<kRowShr8>(cand, kIdent));
-    const int r0 = rdl(cand, 15), r1 = rdl(cand, 31), r2 = rdl(cand, 47), r3 = rdl(cand, 63);
-    return L.lane >= 32 ? min(r2, r3) : min(r0, r1);
+    return min(half_bcast<15>(cand), half_bcast<31>(cand));
   } else {
     int res = CAP;
 #pragma unroll
