@@ -1893,10 +1893,10 @@ static hipError_t l_philox(const void* i, void* o, int n, int, hipStream_t s) {
 }
 
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form) {
-  if (!in || !out || form < 0 || form > 4 || n <= 0 ||
-      n > (form == 1 ? 1024 : form == 2 ? 128 : form == 3 ? 256 : form == 4 ? 128 : 64))
+  if (!in || !out || form < 0 || form > 5 || n <= 0 ||
+      n > (form == 1 ? 1024 : form == 2 ? 128 : form == 3 ? 256 : form == 4 ? 128 : form == 5 ? 512 : 64))
     return fail(DSM_ERR_ARG, "bad probe arguments");
-  const int chains = form == 1 ? 1 : (form == 2 || form == 3) ? 2 : 4;
+  const int chains = (form == 1 || form == 5) ? 1 : (form == 2 || form == 3) ? 2 : 4;
   const int nin = chains * n, nout = chains * (n + 1);
   return probe_run(device_id, in, 4 * (size_t)nin, out, 4 * (size_t)nout, l_scan, n, form);
 }

@@ -687,7 +687,22 @@ __global__ void __launch_bounds__(64) com_probe_kernel(const float* in, float* o
 }
 
 __global__ void __launch_bounds__(64) scan_probe_kernel(const float* in, float* out, int n, int form) {
-  if (form == 1) {
+  if (form == 5) {
+    const LaneC<1> L;
+    float v[8], g[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = L.pos(j);
+      v[j] = p < n ? in[p] : 0.0f;
+    }
+    const float tot = chain_scan<1, 2>(v, g, n, L);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = L.pos(j);
+      if (p < n) out[p] = g[j];
+    }
+    if (L.lane == 0) out[n] = tot;
+  } else if (form == 1) {
     const LaneC<1> L;
     float v[16], g[16];
 #pragma unroll

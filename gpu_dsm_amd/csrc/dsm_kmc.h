@@ -364,6 +364,34 @@ __device__ __forceinline__ float chain_scan(const float (&v)[4 * G], float (&out
 #pragma unroll
     for (int k = 0; k < 4; ++k) out[k] = upper ? B0 + I[k] : I[k];
     return Z > 64 ? B0 + B1 : B0;
+  } else if constexpr (G == 2) {
+    // the serial carry over the four rows of each group: the row totals come from two
+    // 32-lane swizzles (lower half: blocks 4g and 4g+1, upper half: 4g+2 and 4g+3), the
+    // carry through block 4g+1 crosses to the upper half and the carry through block
+    // 4g+3 to the next group by v_readlane; the additions are the same, in block order
+    const int nb = ((__builtin_amdgcn_readfirstlane(Z) - 1) >> 6) + 1;
+    const int row = L.lane >> 4;
+    const bool up = L.lane >= 32, odd = (L.lane & 16) != 0;
+    float carry = 0.0f, total = 0.0f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      float I[4];
+      block_scan4(v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3], I);
+      const float lo = half_bcast<15>(I[3]), hi = half_bcast<31>(I[3]);
+      const float S0 = (g == 0) ? lo : carry + lo;  // through block 4g (lower half)
+      const float S1 = S0 + hi;                     // through block 4g+1 (lower half)
+      const float S1u = rdl(S1, 0);
+      const float S2 = S1u + lo;                    // through block 4g+2 (upper half)
+      const float S3 = S2 + hi;                     // through block 4g+3 (upper half)
+      const float cin = up ? (odd ? S2 : S1u) : (odd ? S0 : carry);   // into the lane's row
+      const float cout = up ? (odd ? S3 : S2) : (odd ? S1 : S0);      // through it
+      const int last = nb - 1 - 4 * g;
+      if (last >= 0 && last < 4) total = rdl(cout, 16 * last);
+      carry = rdl(S3, 32);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out[4 * g + k] = (g == 0 && row == 0) ? I[k] : cin + I[k];
+    }
+    return total;
   } else {
     const int nb = ((__builtin_amdgcn_readfirstlane(Z) - 1) >> 6) + 1;
     const int row = L.lane >> 4;

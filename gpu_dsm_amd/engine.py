@@ -414,12 +414,13 @@ def probe_scan(values, form=1, device=0):
     sums, total).  form 0: values shaped (4, n<=64), four row-form chains -> (sums (4, n),
     totals (4,)); form 2: values shaped (2, n<=128), two pair-form chains; form 3: (2, n<=256),
     two chains of the pair form with two register groups; form 4: (4, n<=128), four chains of
-    the row form with two register groups."""
+    the row form with two register groups; form 5: as form 1 (len <= 512) in the wave form
+    with two register groups."""
     v = np.ascontiguousarray(values, np.float32)
-    if form == 1:
+    if form in (1, 5):
         n = v.size
         out = np.zeros(n + 1, np.float32)
-        _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, 1))
+        _check(load_library().dsm_probe_scan(device, _p(v, C.c_float), _p(out, C.c_float), n, int(form)))
         return out[:n], out[n]
     rows = 2 if form in (2, 3) else 4  # forms 0 and 4: four chains
     v = v.reshape(rows, -1)

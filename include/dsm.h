@@ -233,7 +233,8 @@ int dsm_ensemble_scalars(dsm_engine* e, int32_t ens, float* out24);
    n <= 64 positions in the row form, in = 4*n values, out = 4*(n+1); form 2: two
    chains of n <= 128 positions in the pair form, in = 2*n, out = 2*(n+1); form 3: two
    chains of n <= 256 positions in the pair form with two register groups; form 4: four
-   chains of n <= 128 positions in the row form with two register groups),
+   chains of n <= 128 positions in the row form with two register groups; form 5: one
+   chain of n <= 512 positions in the wave form with two register groups, as form 1),
    exp/log/sin/cos (which 0..3), Philox (6 words in, 4 out). */
 int dsm_probe_scan(int32_t device_id, const float* in, float* out, int32_t n, int32_t form);
 int dsm_probe_math(int32_t device_id, const float* in, float* out, int32_t n, int32_t which);

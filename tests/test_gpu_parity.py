@@ -62,6 +62,16 @@ def test_scan_tree_matches_oracle(oracle):
             gpu, tot = g.probe_scan(vals, form=1)
             assert np.array_equal(_bits(gpu), _bits(oracle.scan_tree(vals)))
             assert _bits(np.float32(tot)) == _bits(np.float32(oracle.tree_total(vals)))
+            if n <= 512:
+                # the wave form with two register groups (its own carry code)
+                gpu, tot = g.probe_scan(vals, form=5)
+                assert np.array_equal(_bits(gpu), _bits(oracle.scan_tree(vals)))
+                assert _bits(np.float32(tot)) == _bits(np.float32(oracle.tree_total(vals)))
+    for n in (2, 100, 128, 129, 191, 192, 193, 255, 320, 384, 385, 449, 511, 512):
+        vals = (rng.exponential(size=n) * 20).astype(np.float32)
+        gpu, tot = g.probe_scan(vals, form=5)
+        assert np.array_equal(_bits(gpu), _bits(oracle.scan_tree(vals)))
+        assert _bits(np.float32(tot)) == _bits(np.float32(oracle.tree_total(vals)))
     for n in (1, 5, 40, 63, 64):
         vals = (rng.standard_normal((4, n)) * 3).astype(np.float32)
         gpu, tot = g.probe_scan(vals, form=0)
